@@ -1,0 +1,160 @@
+"""Benchmark of the evolution hot path on one MI355X.
+
+Metric: grid-point updates per second, FP64: the sum over levels and level
+steps of Nrtotal*Nztotal divided by wall time, state resident in HBM.  A
+"step" is one coarse Berger-Oliger step of the workload (all levels).
+
+Default workload: the Brill-wave configuration (5 box-in-box levels of
+256x512, ICN, 1+log + Gamma-driver 2), synthetic data from the seeded par
+generator (Brill q function with psi = 1).
+
+Prints one JSON line.  Only one GPU is supported: --gpus above 1 exits with
+a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ollinaxis-bib_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, TFLOP/s
+
+
+def workload_text(name, seed):
+    from oaxis import workloads
+    if name == "brill":
+        return workloads.brill(seed), "brill: BrillWave a~U[1,4] (seed %d), 5 levels of 256x512, ICN, " \
+                                      "1+log + Gammadriver2, psi=1 synthetic" % seed
+    if name == "plain_small":
+        return workloads.plain_small(seed), "plain_small: Schwarzschild M=1, 1 level 128x256, RK4, 1+log"
+    if name == "plain_256x512":
+        return workloads.plain_small(seed, 256, 512), "plain: Schwarzschild M=1, 1 level 256x512, RK4, 1+log"
+    if name == "plain_512x1024":
+        return workloads.plain_small(seed, 512, 1024), "plain: Schwarzschild M=1, 1 level 512x1024, RK4, 1+log"
+    raise SystemExit("unknown workload %s" % name)
+
+
+def bytes_per_update(params, nfields):
+    """Algorithmic HBM bytes per grid-point update: RK4 136 N, ICN-3 64 N
+    (reads of y, y_p, acc and writes of acc, y per substep, 8 B each)."""
+    if params.integrator == 1:       # rk4
+        return 136.0 * nfields, 4
+    it = params.icniter
+    return 8.0 * nfields * (2 + 3 * (it - 1)), it
+
+
+def cpu_baseline(text, steps):
+    """The oracle (plain C restatement, one core) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oaxis
+    import pyoracle
+    p = oaxis.params_from_text(text)
+    orc = pyoracle.Oracle(p)
+    orc.initial_data()
+    orc.initial_aux()
+    d = orc.dims(0)
+    per_coarse = sum(2 ** l for l in range(orc.nlevels)) * d["Nr"] * d["Nz"]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.coarse_step()
+    dt = time.perf_counter() - t0
+    orc.close()
+    return per_coarse * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed coarse steps")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed coarse steps")
+    ap.add_argument("--workload", default="brill")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (every level of a spacetime stays resident on "
+              "one device; batched instances shard by HBM footprint); --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import oaxis
+    from oaxis import workloads
+    text, desc = workload_text(args.workload, args.seed)
+    params = oaxis.params_from_text(text)
+    shift = params.shift != oaxis.ENUMS["OA_SHIFT_NONE"]
+    scalar = bool(params.mattertype & oaxis.ENUMS["OA_MATTER_SCALAR"])
+    nfields = workloads.evolved_count(shift, scalar)
+
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        have_torch = False
+
+    st = oaxis.Spacetime(params)
+    st.initial_data()
+    st.initial_aux()
+    if args.warmup > 0:
+        st.evolve(args.warmup)
+    u0 = st.point_updates()
+    st.set_kernel_timing(True)
+    if have_torch:
+        torch.cuda.synchronize()
+    st.sync()
+    t0 = time.perf_counter()
+    st.evolve(args.steps)
+    st.sync()
+    if have_torch:
+        torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    updates = st.point_updates() - u0
+    rhs_ms, rhs_n = st.rhs_kernel_stats()
+    device_ms = st.last_elapsed_ms()
+    value = updates / wall
+
+    bpu, niter = bytes_per_update(params, nfields)
+    # algorithmic bytes per RHS launch: per-update bytes over the substeps of a
+    # step, times the points one launch advances (all levels have the same size)
+    points_per_launch = float(params.Nrtotal) * float(st.level_info(0)["Nz"])
+    bytes_per_launch = bpu / niter * points_per_launch
+    achieved = bytes_per_launch / (rhs_ms * 1e-3) / 1e9 if rhs_ms > 0 else 0.0
+
+    rec = {
+        "metric": "grid-point updates/s (FP64)",
+        "value": value,
+        "unit": "grid-point updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded par generator, seed %d)" % args.seed,
+        "config": {"workload": desc, "path": "plain (one spacetime)", "levels": st.nlevels,
+                   "Nrtotal": params.Nrtotal, "Nztotal": params.Nztotal, "evolved_fields": nfields,
+                   "integrator": "rk4" if params.integrator == 1 else "icn",
+                   "point_updates": updates, "device_ms": device_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_rhs", "avg_launch_ms": rhs_ms, "launches": rhs_n,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    st.close()
+    if not args.no_cpu_baseline:
+        v, secs = cpu_baseline(text, args.cpu_steps)
+        rec["cpu_baseline"] = {"value": v, "unit": "grid-point updates/s", "cores": 1, "kind": "port",
+                               "sample": "%d coarse step(s) of the same workload on the oracle (C restatement, "
+                                         "1 thread), %.1f s" % (args.cpu_steps, secs)}
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

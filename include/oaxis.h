@@ -1,0 +1,233 @@
+/*
+ * oaxis.h -- C ABI of the MI355X (gfx950) evolution hot path for OllinAxis-BiB.
+ *
+ * The library replaces the body of the reference's Berger-Oliger step
+ * (src/base/onestep.f90:78-469) -- sources, boundary conditions, RK4/ICN
+ * accumulate/update, fine-boundary fill, axis/equator symmetries, the
+ * state-mutating part of "auxiliary", restriction -- with HIP kernels that keep
+ * every refinement level resident in device memory.  A Fortran main loop calls
+ * it through ISO_C_BINDING (see ollinaxis-bib_amd/fortran/oaxis_mod.f90).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a nonzero OA_ERR_* code on
+ *     failure; oa_last_error() returns a message for the last failure on the
+ *     calling thread.  Nothing aborts the process.
+ *   - Host arrays exchanged with the library use the reference's layout: Fortran
+ *     column-major, rho index fastest, extent (1-ghost:Nr, 1-ghost:Nz) of the
+ *     level, leading dimension ld >= Nr+ghost (src/base/arrays.config, generated
+ *     allocation in prl/arrays.pl:636-646).  Device storage is z-fastest; the
+ *     transposition happens inside upload/download.
+ *   - Field names are the reference's evolved-array names (arrays.config):
+ *     alpha, phi, chi, A, B, H, C, trK, KTA, KTB, KTH, KTC, Delta_r, Delta_z,
+ *     lambda, Alambda, beta_r, beta_z, dtbeta_r, dtbeta_z, scalar_phi,
+ *     scalar_xi_r, scalar_xi_z, scalar_pi; plus the derived psi, psi4.
+ *   - A handle is not re-entrant; distinct handles are independent.
+ */
+#ifndef OAXIS_H
+#define OAXIS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+#define OA_OK              0
+#define OA_ERR_ARG         1   /* bad argument / unknown name            */
+#define OA_ERR_PARAM       2   /* par-file error (syntax, range, type)   */
+#define OA_ERR_UNSUPPORTED 3   /* valid reference setting outside scope  */
+#define OA_ERR_HIP         4   /* HIP runtime failure                    */
+#define OA_ERR_STATE       5   /* call out of order                      */
+#define OA_ERR_NOMEM       6   /* device or host allocation failed       */
+#define OA_ERR_IO          7   /* file could not be read                 */
+
+/* ------------------------------------------------------- parameter values */
+/* String-valued reference keys map to these codes (param.f90 ranges). */
+enum { OA_ORDER_TWO = 2, OA_ORDER_FOUR = 4 };
+enum { OA_ICN = 0, OA_RK4 = 1 };
+enum { OA_SLICING_STATIC = 0, OA_SLICING_HARMONIC, OA_SLICING_1PLUSLOG,
+       OA_SLICING_SHOCKAVOID, OA_SLICING_HARMONIC2, OA_SLICING_1PLUSLOG2,
+       OA_SLICING_SHOCKAVOID2, OA_SLICING_MAXIMAL };
+enum { OA_ILAPSE_ONE = 0, OA_ILAPSE_SHELL, OA_ILAPSE_TORUS, OA_ILAPSE_ISOTROPIC,
+       OA_ILAPSE_QUISO_KERR, OA_ILAPSE_PSIMINUS2, OA_ILAPSE_PSIMINUS4,
+       OA_ILAPSE_MAXIMAL };
+enum { OA_SHIFT_NONE = 0, OA_SHIFT_ZERO, OA_SHIFT_STATIC, OA_SHIFT_GD0,
+       OA_SHIFT_GD1, OA_SHIFT_GD2, OA_SHIFT_GDSHOCK1, OA_SHIFT_GDSHOCK2 };
+enum { OA_IDATA_CHECKPOINT = 0, OA_IDATA_MINKOWSKI, OA_IDATA_SCHWARZSCHILD,
+       OA_IDATA_KERR, OA_IDATA_BRILLLINDQUIST, OA_IDATA_BRILLWAVE,
+       OA_IDATA_SCALARPULSE, OA_IDATA_COMPLEXPULSE, OA_IDATA_TESTGW,
+       OA_IDATA_BOSONSTAR, OA_IDATA_ROTBOSON };
+enum { OA_BOUND_NONE = 0, OA_BOUND_STATIC, OA_BOUND_FLAT, OA_BOUND_RADIATIVE };
+enum { OA_INTERP_BILINEAR = 1, OA_INTERP_BICUBIC = 3 };
+enum { OA_EULERIAN = 0, OA_LAGRANGIAN = 1 };
+enum { OA_BSSN = 0, OA_Z4C = 1 };
+enum { OA_DYNAMIC = 0, OA_BACKGROUND = 1 };
+enum { OA_PERT_NONE = 0, OA_PERT_GAUSS = 1 };
+enum { OA_ISHIFT_ZERO = 0, OA_ISHIFT_SHELL, OA_ISHIFT_TORUS, OA_ISHIFT_QUISO_KERR };
+enum { OA_POT_NONE = 0, OA_POT_PHI2, OA_POT_PHI4 };
+enum { OA_PULSE_SHELL = 0, OA_PULSE_TORUS, OA_PULSE_BAUMGARTE };
+enum { OA_SCALARMETHOD_FIRST = 1, OA_SCALARMETHOD_SECOND = 2 };
+/* mattertype is a "multiple" key: bit mask */
+enum { OA_MATTER_SCALAR = 1, OA_MATTER_COMPLEX = 2 };
+
+/* Parameters of the evolution hot path.  Member names are the reference's
+ * par-file keys and defaults are the reference's (src/base/param.f90:95-800).
+ * Keys of the reference that this library does not act on (output, horizon
+ * finder tuning, elliptic solver, complex field, ...) are still accepted by the
+ * parser so that reference par files load unchanged. */
+typedef struct oa_params {
+  /* grid (param.f90:95-168) */
+  int    eqsym;
+  double dr0, dz0;
+  int    Nrtotal, Nztotal;
+  int    Nb;
+  int    Nl0, Nl1, Nl2, Nl3;
+  int    ghost;                   /* 0: chosen from "order" (main.f90:286-317) */
+  int    interporder;             /* OA_INTERP_*                            */
+  double ft1, ft2, ft3, ft4, ft5, ft6, ft7, ft8;
+  /* time stepping (param.f90:209-221) */
+  int    adjuststep;
+  double dtfac;
+  int    Nt;
+  /* slicing (param.f90:289-303) */
+  int    slicing, ilapse;
+  int    maximalevery;
+  double gauge_f, gauge_eta;
+  int    lapsepert;
+  double lapse_a0, lapse_r0, lapse_z0, lapse_sr0, lapse_sz0;
+  /* shift (param.f90:330-349) */
+  int    shift, ishift;
+  int    driverD0, driveradv;
+  double shiftafter, drivercsi, drivereta;
+  int    shiftpert;
+  double shift_a0, shift_r0, shift_z0, shift_sr0, shift_sz0;
+  /* initial data (param.f90:359-361) */
+  int    idata;
+  int    factorhdet;
+  /* evolution (param.f90:400-438) */
+  int    spacetime, formulation, bssnflavor, integrator, order;
+  int    angmom, chimethod, evolveH, evolveKTH, shiftadvect;
+  int    nolambda, noDelta_r, noDelta_z, noDelta_p;
+  int    icniter, chipower;
+  double eta, kappa1, kappa2;
+  double geodiss, scalardiss, complexdiss;
+  int    boundtype;
+  /* black holes (param.f90:491-502) */
+  int    N_BH;
+  double BH1mass, BH1spin, BH1Z, BH2mass, BH2spin, BH2Z;
+  int    duplicate_bhs;
+  /* horizons: only what the serving loop uses (param.f90:540-562) */
+  int    ahfind, ahfind_every;
+  double ahafter;
+  /* Brill waves (param.f90:572-576) */
+  double brill_a0, brill_sr0, brill_sz0;
+  /* matter (param.f90:615-648) */
+  int    mattertype;              /* OA_MATTER_* bit mask, 0 = vacuum       */
+  int    scalarpotential, scalarpulse_type, scalarmethod;
+  double scalar_mass, scalar_lambda;
+  double scalar_a0, scalar_r0, scalar_z0, scalar_sr0, scalar_sz0;
+} oa_params;
+
+/* Fill *p with the reference defaults. */
+int    oa_params_default(oa_params* p);
+/* Assign one "key = value" pair with the reference's typing and range rules
+ * (generated assign.f90, prl/assign.pl:28-259).  Unknown key -> OA_ERR_PARAM. */
+int    oa_params_set(oa_params* p, const char* key, const char* value);
+/* Parse a whole par file ("key = value", '#' comments; parse.f90:7-373). */
+int    oa_params_parse_file(oa_params* p, const char* path);
+/* Parse par-file text held in memory (same rules as the file form). */
+int    oa_params_parse_text(oa_params* p, const char* text);
+/* Read back one key as text (for tests / drivers without struct access). */
+int    oa_params_get(const oa_params* p, const char* key, char* out, int outlen);
+/* Check that the settings lie inside what this library implements. */
+int    oa_params_check_supported(const oa_params* p, char* msg, int msglen);
+size_t oa_params_size(void);
+
+const char* oa_last_error(void);
+
+/* --------------------------------------------------------------- handles */
+typedef struct oa_handle oa_handle;
+
+/* Allocate every level of box 0 on the current HIP device.  Grid geometry,
+ * time steps and the forced odd z size follow setupboxes.f90:104-141 and
+ * evolve.f90:42-72. */
+int oa_create(const oa_params* p, oa_handle** h);
+int oa_destroy(oa_handle* h);
+
+/* Level geometry: number of levels, and per level the local sizes Nr, Nz
+ * (interior + boundary, ghosts excluded), ghost, dr, dz, dt. */
+int oa_num_levels(const oa_handle* h, int* nlevels);
+int oa_level_info(const oa_handle* h, int box, int level, int* Nr, int* Nz,
+                  int* ghost, double* dr, double* dz, double* dt);
+
+/* Host <-> device transfer of one named array, reference layout (see top). */
+int oa_upload_field(oa_handle* h, int box, int level, const char* name,
+                    const double* src, int ld);
+int oa_download_field(oa_handle* h, int box, int level, const char* name,
+                      double* dst, int ld);
+
+/* Level clocks t, t1, t2 and step counter s (onestep.f90:351-357). */
+int oa_get_time(const oa_handle* h, int box, int level, double* t, double* t1,
+                double* t2, int* s);
+int oa_set_time(oa_handle* h, int box, int level, double t, double t1,
+                double t2, int s);
+
+/* ---- the Berger-Oliger step body, callable piecewise in the reference order
+ * (onestep.f90:97, 188-335, 351-357, 421-463). */
+/* saveold: keep the previous-time state and shift the fine-boundary time
+ * levels (generated saveold, prl/arrays.pl:759-875). */
+int oa_saveold(oa_handle* h, int box, int level);
+/* One internal iteration: sources_geometry/lapse/shift/matter, radiative
+ * boundaries (level 0), accumulate (RK4), update, axis/equator symmetries and
+ * the pointwise part of auxiliary (factordet, KT_H, lambda, chi/phi).
+ * iter in 1..niter, dtw and weight as in onestep.f90:141-177. */
+int oa_substep(oa_handle* h, int box, int level, int iter, int niter,
+               double dtw, double weight);
+/* finebound(box, level, dtw, .true.) for level > 0 (finebound.f90:2-407). */
+int oa_finebound(oa_handle* h, int box, int level, double dtw);
+/* auxiliary: completes the state projection that needs neighbours
+ * (Delta = DelDef while the level clock is 0) and re-projects a level whose
+ * data was changed by restriction or upload (auxiliary_geometry.f90). */
+int oa_auxiliary(oa_handle* h, int box, int level);
+/* restrict(box, level, .true.) into level-1 (restrict.f90:2-169) followed by
+ * that level's symmetries (onestep.f90:431-451). */
+int oa_restrict(oa_handle* h, int box, int level);
+/* t2=t1; t1=t; s=s+1; t=t+dt (onestep.f90:351-357). */
+int oa_advance_time(oa_handle* h, int box, int level);
+
+/* ---- whole-step entry points */
+/* evolve.f90:128-195: symmetries and auxiliary on every level, then the
+ * Gamma-driver-2 initialisation dtbeta = drivercsi * Delta. */
+int oa_initial_aux(oa_handle* h);
+/* onestep(0) including the recursion into finer levels and restriction. */
+int oa_coarse_step(oa_handle* h);
+/* nsteps coarse steps. */
+int oa_evolve(oa_handle* h, int nsteps);
+/* Wait for all device work of the handle. */
+int oa_sync(oa_handle* h);
+
+/* Closed-form initial data on every level (host side, not the hot path):
+ * minkowski (+ gauss lapse/shift perturbations), schwarzschild,
+ * BrillLindquist, scalarpulse with psi = 1 (no Hamiltonian-constraint
+ * solve).  Follows initial.f90:53-150,189-330,528-620 and
+ * idata_scalarpulse.f90:60-103. */
+int oa_initial_data(oa_handle* h);
+
+/* Device time of the last oa_evolve / oa_coarse_step call in ms (HIP events
+ * on the handle's stream), and the kernel launch counts. */
+int oa_last_elapsed_ms(const oa_handle* h, double* ms);
+/* Per-launch HIP-event timing of the dominant RHS kernel during oa_evolve
+ * (off by default; adds two event records per launch on the same stream). */
+int oa_set_kernel_timing(oa_handle* h, int on);
+/* Average duration (ms) of the dominant RHS kernel over the last timed
+ * interval and how many launches it averaged. */
+int oa_rhs_kernel_stats(const oa_handle* h, double* avg_ms, long* launches);
+/* Grid-point updates (sum over levels and steps of Nrtotal*Nztotal) done
+ * by this handle so far. */
+int oa_point_updates(const oa_handle* h, double* updates);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OAXIS_H */

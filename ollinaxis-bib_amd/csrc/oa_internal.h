@@ -1,0 +1,93 @@
+// Internal declarations shared by the host runtime and the HIP kernels.
+#pragma once
+
+#include <string>
+
+#include "oaxis.h"
+
+// thread-local last-error text behind oa_last_error()
+void oa_set_error(const std::string& msg);
+
+namespace oax {
+
+// Device state slots.  Every level stores its evolved fields as z-fastest
+// planes, one plane per field, fields of one slab contiguous.  Vacuum uses
+// the first NBASE slots; the Gamma-driver shift and the real scalar field
+// append their slots in that order.
+enum Slot {
+  S_alpha = 0, S_chi, S_phi, S_A, S_B, S_H, S_C, S_trK, S_KTA, S_KTB, S_KTC,
+  S_Delta_r, S_Delta_z,
+  NBASE
+};
+enum ShiftSlot { SH_beta_r = 0, SH_beta_z, SH_dtbeta_r, SH_dtbeta_z, NSHIFT };
+enum ScalarSlot { SC_phi = 0, SC_xi_r, SC_xi_z, SC_pi, NSCALAR };
+
+constexpr int nslots(bool shift, bool scalar) {
+  return NBASE + (shift ? NSHIFT : 0) + (scalar ? NSCALAR : 0);
+}
+constexpr int shift_slot(int s) { return NBASE + s; }
+constexpr int scalar_slot(bool shift, int s) { return NBASE + (shift ? NSHIFT : 0) + s; }
+
+// Geometry of one level as the kernels see it.  Point (i, j) of the
+// reference's (1-g:Nr, 1-g:Nz) range lives at (i+g-1)*ld + (j+g-1).
+struct KGrid {
+  int Nr, Nz, g, ld;
+  long plane;           // elements of one field plane (nr * ld)
+  int eqsym;
+  int jlo;              // first computed z index: 1 with equatorial symmetry, 1-g without
+  double dr, dz, dt;
+  double idr, idz;      // 1/dr, 1/dz
+  double zoff;          // z_j = (j - zoff) * dz
+};
+
+// Physics switches and constants of one right-hand-side launch.
+struct KPhys {
+  int slicing;
+  double gauge_f, gauge_eta;
+  double geodiss, scalardiss, eta;
+  double drivercsi, drivereta;
+  int driveradv, shiftadvect;
+  int scalarpotential, scalarmethod;
+  double scalar_mass, scalar_lambda;
+  int chimethod;
+  int radiative;        // level 0 with boundtype = radiative
+};
+
+// RK bookkeeping of one internal iteration.
+struct KStep {
+  int iter, niter, rk4;
+  double dtw, w;
+  int proj_t0;          // the auxiliary call ending this iteration runs at clock 0
+};
+
+// Time-interpolation coefficients of one finebound call (computed on the
+// host exactly as the generated bound_interp.inc forms them).
+struct KTime {
+  int border;           // 1: linear with var_p, 2: quadratic with strips
+  int depth;            // imax = jmax
+  int same;             // tp == tl: strip terms carry zero weight
+  double c_aux, c_p;    // border 1
+  double c0, c1, c2;    // border 2: coarse value, strip t0, strip tm1
+};
+
+// launchers (oaxis_kernels.hip)
+int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar,
+               bool phimode, const double* yin, const double* yp, double* acc, double* yout,
+               void* stream);
+int launch_project(const KGrid& G, const KPhys& P, bool shift, bool scalar, bool time0,
+                   double* y, void* stream);
+int launch_deldef(const KGrid& G, double* y, void* stream);
+int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream);
+int launch_saveold(const KGrid& G, int nf, const double* y, double* strips_rR_t0,
+                   double* strips_zL_t0, double* strips_zR_t0, void* stream);
+int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T,
+                     bool shift, bool scalar, bool time0, const double* ycoarse,
+                     double* yfine, const double* rR0, const double* rR1,
+                     const double* zL0, const double* zL1, const double* zR0,
+                     const double* zR1, int interporder, void* stream);
+int launch_restrict(const KGrid& Gf, const KGrid& Gc, int nf, int interporder,
+                    const double* yfine, double* ycoarse, void* stream);
+int launch_transpose_in(const KGrid& G, const double* host_colmajor_dev, int ld_src,
+                        double* plane, int sym_r, int sym_z, void* stream);
+
+}  // namespace oax

@@ -1,0 +1,670 @@
+// Host runtime of the evolution hot path: device residency of every level of
+// box 0, the Berger-Oliger schedule of onestep.f90, host <-> device transfer
+// in the reference layout, and the C ABI declared in include/oaxis.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "oa_internal.h"
+#include "oaxis.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void oa_set_error(const std::string& msg) { g_last_error = msg; }
+extern "C" const char* oa_last_error(void) { return g_last_error.c_str(); }
+
+namespace oax {
+
+struct Level {
+  KGrid G{};
+  double* slab[3] = {nullptr, nullptr, nullptr};   // state buffers
+  double* acc = nullptr;                            // RK4 accumulator
+  double* strip[3] = {nullptr, nullptr, nullptr};  // fine-boundary time levels (ring)
+  int ring[3] = {0, 1, 2};                          // ring[k]: strip set of time level -k
+  int cur = 0;                                      // slab holding the current state
+  int step_a = 0, last_out = 0;                     // roles inside a step
+  bool in_step = false;
+  bool dirty = true;                                // state needs the full projection
+  bool phi_valid = true;                            // phi plane holds the reference's phi
+  bool aux_phi = true;                              // last auxiliary ran at clock 0 (phi mode)
+  double t = 0.0, t1 = 0.0, t2 = 0.0;
+  int s = 0;
+  long strip_r = 0, strip_z = 0;                    // doubles per strip set piece
+};
+
+}  // namespace oax
+
+using namespace oax;
+
+struct oa_handle {
+  oa_params p{};
+  int nlev = 0;
+  int nf = 0;
+  bool shift = false, scalar = false;
+  double dt0 = 0.0;
+  std::vector<Level> L;
+  hipStream_t stream = nullptr;
+  KPhys phys{};
+  // timing
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  double last_ms = 0.0;
+  bool kernel_timing = false;
+  std::vector<hipEvent_t> kev;     // pairs around k_rhs launches
+  size_t kev_used = 0;
+  double rhs_avg_ms = 0.0;
+  long rhs_launches = 0;
+  double point_updates = 0.0;
+};
+
+namespace {
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return OA_OK;
+  oa_set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return OA_ERR_HIP;
+}
+
+#define HIPCHK(call)                                  \
+  do {                                                \
+    int rc_ = hip_check((call), #call);               \
+    if (rc_ != OA_OK) return rc_;                     \
+  } while (0)
+
+#define RC(call)                                      \
+  do {                                                \
+    int rc_ = (call);                                 \
+    if (rc_ != OA_OK) return rc_;                     \
+  } while (0)
+
+struct FieldName {
+  const char* name;
+  int slot;       // device slot, -1 = derived
+  int symr, symz;
+};
+
+int slot_of(const oa_handle* h, const std::string& name) {
+  static const char* base[] = {"alpha", "chi", "phi", "A", "B", "H", "C", "trK", "KTA", "KTB", "KTC",
+                               "Delta_r", "Delta_z"};
+  for (int s = 0; s < NBASE; ++s)
+    if (name == base[s]) return s;
+  if (h->shift) {
+    static const char* sh[] = {"beta_r", "beta_z", "dtbeta_r", "dtbeta_z"};
+    for (int s = 0; s < NSHIFT; ++s)
+      if (name == sh[s]) return shift_slot(s);
+  }
+  if (h->scalar) {
+    static const char* sc[] = {"scalar_phi", "scalar_xi_r", "scalar_xi_z", "scalar_pi"};
+    for (int s = 0; s < NSCALAR; ++s)
+      if (name == sc[s]) return scalar_slot(h->shift, s);
+  }
+  return -1;
+}
+
+void fill_phys(oa_handle* h) {
+  const oa_params& p = h->p;
+  KPhys& P = h->phys;
+  P.slicing = p.slicing;
+  P.gauge_f = p.gauge_f;
+  P.gauge_eta = p.gauge_eta;
+  P.geodiss = p.geodiss;
+  P.scalardiss = p.scalardiss;
+  P.eta = p.eta;
+  P.drivercsi = p.drivercsi;
+  P.drivereta = p.drivereta;
+  P.driveradv = p.driveradv;
+  P.shiftadvect = p.shiftadvect;
+  P.scalarpotential = p.scalarpotential;
+  P.scalarmethod = p.scalarmethod;
+  P.scalar_mass = p.scalar_mass;
+  P.scalar_lambda = p.scalar_lambda;
+  P.chimethod = p.chimethod;
+  P.radiative = 0;
+}
+
+Level* level_of(oa_handle* h, int box, int level) {
+  if (!h || box != 0 || level < 0 || level >= h->nlev) {
+    oa_set_error("bad box/level (only box 0 exists; level out of range)");
+    return nullptr;
+  }
+  return &h->L[level];
+}
+
+double* strip_ptr(oa_handle* h, Level& L, int tl, int piece) {
+  // piece 0: rR, 1: zL, 2: zR
+  double* base = L.strip[L.ring[tl]];
+  const long nf = h->nf;
+  if (piece == 0) return base;
+  if (piece == 1) return base + nf * L.strip_r;
+  return base + nf * L.strip_r + nf * L.strip_z;
+}
+
+// ------------------------------------------------------------ step pieces
+int do_saveold(oa_handle* h, int level) {
+  Level& L = h->L[level];
+  // time levels shift: t-2 <- t-1 <- t0 <- new copy
+  const int oldest = L.ring[2];
+  L.ring[2] = L.ring[1];
+  L.ring[1] = L.ring[0];
+  L.ring[0] = oldest;
+  RC(launch_saveold(L.G, h->nf, L.slab[L.cur], strip_ptr(h, L, 0, 0), strip_ptr(h, L, 0, 1),
+                    strip_ptr(h, L, 0, 2), h->stream));
+  L.step_a = L.cur;
+  L.last_out = L.cur;
+  L.in_step = true;
+  return OA_OK;
+}
+
+int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double w) {
+  Level& L = h->L[level];
+  if (!L.in_step) {
+    oa_set_error("oa_substep called before oa_saveold");
+    return OA_ERR_STATE;
+  }
+  if (L.dirty) {
+    oa_set_error("oa_substep on a level whose state is not projected (call oa_auxiliary / oa_initial_aux)");
+    return OA_ERR_STATE;
+  }
+  const int a = L.step_a;
+  const int in = (iter == 1) ? a : L.last_out;
+  int out;
+  if (iter == niter && in != a) {
+    out = a;
+  } else {
+    out = 0;
+    while (out == a || out == in) ++out;
+  }
+  const bool time0 = (L.t == 0.0);
+  const bool phimode = L.aux_phi;
+  KPhys P = h->phys;
+  P.radiative = (level == 0 && h->p.boundtype == OA_BOUND_RADIATIVE) ? 1 : 0;
+  KStep S{iter, niter, h->p.integrator == OA_RK4 ? 1 : 0, dtw, w, time0 ? 1 : 0};
+  if (h->kernel_timing) {
+    if (h->kev_used + 2 > h->kev.size()) {
+      for (int q = 0; q < 256; ++q) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        h->kev.push_back(e);
+      }
+    }
+    HIPCHK(hipEventRecord(h->kev[h->kev_used], h->stream));
+  }
+  RC(launch_rhs(L.G, P, S, h->shift, h->scalar, phimode, L.slab[in], L.slab[a], L.acc, L.slab[out], h->stream));
+  L.aux_phi = time0;
+  if (h->kernel_timing) {
+    HIPCHK(hipEventRecord(h->kev[h->kev_used + 1], h->stream));
+    h->kev_used += 2;
+  }
+  L.last_out = out;
+  if (!time0 && h->p.chimethod) L.phi_valid = false;
+  if (iter == niter) {
+    L.cur = out;
+    L.in_step = false;
+  }
+  return OA_OK;
+}
+
+int do_finebound(oa_handle* h, int level, double dtw) {
+  if (level == 0) return OA_OK;
+  Level& L = h->L[level];
+  Level& Lc = h->L[level - 1];
+  const double tp = Lc.t, t0 = L.t, tm1 = L.t1, tl = t0 + dtw;
+  KTime T{};
+  T.same = (tp == tl) ? 1 : 0;
+  T.depth = T.same ? L.G.g + 2 : L.G.g - 1;
+  T.border = (tm1 == 0.0) ? 1 : 2;
+  // coefficients exactly as bound_interp.inc forms them
+  T.c_aux = (tl - t0) / (tp - t0);
+  T.c_p = (tl - tp) / (t0 - tp);
+  T.c0 = (tl - t0) * (tl - tm1) / ((tp - t0) * (tp - tm1));
+  T.c1 = (tl - tp) * (tl - tm1) / ((t0 - tp) * (t0 - tm1));
+  T.c2 = (tl - tp) * (tl - t0) / ((tm1 - tp) * (tm1 - t0));
+  const bool time0 = (L.t == 0.0);
+  double* yf = L.slab[L.in_step ? L.last_out : L.cur];
+  RC(launch_finebound(L.G, Lc.G, h->phys, T, h->shift, h->scalar, time0, Lc.slab[Lc.cur], yf,
+                      strip_ptr(h, L, 0, 0), strip_ptr(h, L, 1, 0), strip_ptr(h, L, 0, 1), strip_ptr(h, L, 1, 1),
+                      strip_ptr(h, L, 0, 2), strip_ptr(h, L, 1, 2), h->p.interporder, h->stream));
+  return OA_OK;
+}
+
+int do_auxiliary(oa_handle* h, int level) {
+  Level& L = h->L[level];
+  const bool time0 = (L.t == 0.0);
+  double* y = L.slab[L.in_step ? L.last_out : L.cur];
+  if (L.dirty) {
+    RC(launch_project(L.G, h->phys, h->shift, h->scalar, time0, y, h->stream));
+    L.dirty = false;
+    L.aux_phi = time0;
+    if (!time0 && h->p.chimethod) L.phi_valid = false;
+  }
+  if (time0) RC(launch_deldef(L.G, y, h->stream));
+  return OA_OK;
+}
+
+int do_restrict(oa_handle* h, int level) {
+  Level& L = h->L[level];
+  Level& Lc = h->L[level - 1];
+  RC(launch_restrict(L.G, Lc.G, h->nf, h->p.interporder, L.slab[L.cur], Lc.slab[Lc.cur], h->stream));
+  Lc.dirty = true;
+  return OA_OK;
+}
+
+void advance_time(Level& L) {
+  L.t2 = L.t1;
+  L.t1 = L.t;
+  L.s = L.s + 1;
+  L.t = L.t + L.G.dt;
+}
+
+// onestep(level): onestep.f90:78-469 for box 0
+int onestep(oa_handle* h, int level) {
+  Level& L = h->L[level];
+  RC(do_saveold(h, level));
+  const bool rk4 = (h->p.integrator == OA_RK4);
+  const int niter = rk4 ? 4 : h->p.icniter;
+  const bool time0 = (L.t == 0.0);
+  for (int iter = 1; iter <= niter; ++iter) {
+    double dtw, w = 0.0;
+    if (rk4) {
+      switch (iter) {
+        case 1: dtw = 0.5 * L.G.dt; w = 1.0 / 6.0; break;
+        case 2: dtw = 0.5 * L.G.dt; w = 1.0 / 3.0; break;
+        case 3: dtw = L.G.dt; w = 1.0 / 3.0; break;
+        default: dtw = L.G.dt; w = 1.0 / 6.0; break;
+      }
+    } else {
+      dtw = (iter < niter) ? 0.5 * L.G.dt : L.G.dt;
+    }
+    RC(do_substep(h, level, iter, niter, dtw, w));
+    if (level > 0) RC(do_finebound(h, level, dtw));
+    if (time0) RC(launch_deldef(L.G, L.slab[L.last_out], h->stream));
+  }
+  advance_time(L);
+  h->point_updates += static_cast<double>(h->p.Nrtotal) * static_cast<double>(h->p.Nztotal);
+  if (level < h->nlev - 1) {
+    RC(onestep(h, level + 1));
+    RC(onestep(h, level + 1));
+  }
+  if (level > 0 && (L.s % 2) == 0) {
+    RC(do_restrict(h, level));
+    RC(do_auxiliary(h, level - 1));
+  }
+  return OA_OK;
+}
+
+void fill_grid(KGrid& G, const oa_params& p, int level) {
+  const int g = p.ghost;
+  G.Nr = p.Nrtotal;
+  G.Nz = p.Nztotal;
+  G.g = g;
+  const int nzt = G.Nz + g;
+  G.ld = (nzt + 15) / 16 * 16;
+  G.plane = static_cast<long>(G.Nr + g) * G.ld;
+  G.eqsym = p.eqsym;
+  G.jlo = p.eqsym ? 1 : 1 - g;
+  const double scale = static_cast<double>(1 << level);
+  G.dr = p.dr0 / scale;
+  G.dz = p.dz0 / scale;
+  G.idr = 1.0 / G.dr;
+  G.idz = 1.0 / G.dz;
+  G.zoff = p.eqsym ? 0.5 : 0.5 * static_cast<double>(G.Nz - g + 1);
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
+  if (!pin || !out) {
+    oa_set_error("oa_create: null argument");
+    return OA_ERR_ARG;
+  }
+  *out = nullptr;
+  char msg[256];
+  RC(oa_params_check_supported(pin, msg, sizeof msg));
+  oa_handle* h = new oa_handle();
+  h->p = *pin;
+  oa_params& p = h->p;
+  p.ghost = 3;
+  if (!p.eqsym && ((p.Nztotal + p.ghost) % 2 == 0)) p.Nztotal += 1;   // setupboxes.f90:104-108
+  h->shift = (p.shift != OA_SHIFT_NONE);
+  h->scalar = (p.mattertype & OA_MATTER_SCALAR) != 0;
+  h->nf = nslots(h->shift, h->scalar);
+  h->nlev = p.Nl0 + 1;
+  h->dt0 = p.dtfac * std::fmin(p.dr0, p.dz0);
+  fill_phys(h);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    delete h;
+    oa_set_error("no HIP device available: this library runs on the GPU only");
+    return OA_ERR_HIP;
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->ev_begin) != hipSuccess || hipEventCreate(&h->ev_end) != hipSuccess) {
+    delete h;
+    oa_set_error("HIP stream/event creation failed");
+    return OA_ERR_HIP;
+  }
+  h->L.resize(h->nlev);
+  for (int l = 0; l < h->nlev; ++l) {
+    Level& L = h->L[l];
+    fill_grid(L.G, p, l);
+    L.G.dt = h->dt0 / static_cast<double>(1 << l);   // setupboxes.f90:121-125
+    const size_t bytes = sizeof(double) * static_cast<size_t>(h->nf) * L.G.plane;
+    bool ok = true;
+    for (int b = 0; b < 3; ++b) ok = ok && hipMalloc(&L.slab[b], bytes) == hipSuccess;
+    ok = ok && hipMalloc(&L.acc, bytes) == hipSuccess;
+    L.strip_r = static_cast<long>(L.G.g) * (L.G.Nz + L.G.g);
+    L.strip_z = static_cast<long>(L.G.g) * (L.G.Nr + L.G.g);
+    const size_t sbytes = sizeof(double) * static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
+    for (int b = 0; b < 3; ++b) ok = ok && hipMalloc(&L.strip[b], sbytes) == hipSuccess;
+    if (!ok) {
+      oa_destroy(h);
+      oa_set_error("device allocation failed");
+      return OA_ERR_NOMEM;
+    }
+    for (int b = 0; b < 3; ++b) {
+      (void)hipMemsetAsync(L.slab[b], 0, bytes, h->stream);
+      (void)hipMemsetAsync(L.strip[b], 0, sbytes, h->stream);
+    }
+    (void)hipMemsetAsync(L.acc, 0, bytes, h->stream);
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    oa_destroy(h);
+    oa_set_error("initial device clear failed");
+    return OA_ERR_HIP;
+  }
+  *out = h;
+  return OA_OK;
+}
+
+extern "C" int oa_destroy(oa_handle* h) {
+  if (!h) return OA_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (Level& L : h->L) {
+    for (int b = 0; b < 3; ++b) {
+      if (L.slab[b]) (void)hipFree(L.slab[b]);
+      if (L.strip[b]) (void)hipFree(L.strip[b]);
+    }
+    if (L.acc) (void)hipFree(L.acc);
+  }
+  for (hipEvent_t e : h->kev) (void)hipEventDestroy(e);
+  if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);
+  if (h->ev_end) (void)hipEventDestroy(h->ev_end);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return OA_OK;
+}
+
+extern "C" int oa_num_levels(const oa_handle* h, int* n) {
+  if (!h || !n) return OA_ERR_ARG;
+  *n = h->nlev;
+  return OA_OK;
+}
+
+extern "C" int oa_level_info(const oa_handle* h, int box, int level, int* Nr, int* Nz, int* ghost, double* dr,
+                             double* dz, double* dt) {
+  const Level* L = level_of(const_cast<oa_handle*>(h), box, level);
+  if (!L) return OA_ERR_ARG;
+  if (Nr) *Nr = L->G.Nr;
+  if (Nz) *Nz = L->G.Nz;
+  if (ghost) *ghost = L->G.g;
+  if (dr) *dr = L->G.dr;
+  if (dz) *dz = L->G.dz;
+  if (dt) *dt = L->G.dt;
+  return OA_OK;
+}
+
+extern "C" int oa_upload_field(oa_handle* h, int box, int level, const char* name, const double* src, int ld) {
+  Level* L = level_of(h, box, level);
+  if (!L || !name || !src) return OA_ERR_ARG;
+  const int slot = slot_of(h, name);
+  if (slot < 0) {
+    oa_set_error(std::string("oa_upload_field: no evolved array named ") + name + " in this configuration");
+    return OA_ERR_ARG;
+  }
+  const KGrid& G = L->G;
+  const int nrt = G.Nr + G.g, nzt = G.Nz + G.g;
+  if (ld < nrt) {
+    oa_set_error("oa_upload_field: leading dimension smaller than Nr+ghost");
+    return OA_ERR_ARG;
+  }
+  std::vector<double> plane(static_cast<size_t>(G.plane), 0.0);
+  for (int jj = 0; jj < nzt; ++jj)
+    for (int ii = 0; ii < nrt; ++ii) plane[static_cast<size_t>(ii) * G.ld + jj] = src[static_cast<size_t>(jj) * ld + ii];
+  HIPCHK(hipMemcpyAsync(L->slab[L->cur] + static_cast<long>(slot) * G.plane, plane.data(),
+                        sizeof(double) * plane.size(), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  L->dirty = true;
+  if (slot == S_phi) L->phi_valid = true;
+  return OA_OK;
+}
+
+extern "C" int oa_download_field(oa_handle* h, int box, int level, const char* name, double* dst, int ld) {
+  Level* L = level_of(h, box, level);
+  if (!L || !name || !dst) return OA_ERR_ARG;
+  const KGrid& G = L->G;
+  const int nrt = G.Nr + G.g, nzt = G.Nz + G.g;
+  if (ld < nrt) {
+    oa_set_error("oa_download_field: leading dimension smaller than Nr+ghost");
+    return OA_ERR_ARG;
+  }
+  const std::string nm(name);
+  auto fetch = [&](int slot, std::vector<double>& buf) -> int {
+    buf.resize(static_cast<size_t>(G.plane));
+    double* y = L->slab[L->in_step ? L->last_out : L->cur];
+    HIPCHK(hipMemcpyAsync(buf.data(), y + static_cast<long>(slot) * G.plane, sizeof(double) * buf.size(),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return OA_OK;
+  };
+  auto emit = [&](auto fn) {
+    for (int jj = 0; jj < nzt; ++jj)
+      for (int ii = 0; ii < nrt; ++ii) dst[static_cast<size_t>(jj) * ld + ii] = fn(static_cast<size_t>(ii) * G.ld + jj, ii, jj);
+  };
+  const int slot = slot_of(h, nm);
+  if (slot >= 0 && !(slot == S_phi && !L->phi_valid)) {
+    std::vector<double> b;
+    RC(fetch(slot, b));
+    emit([&](size_t q, int, int) { return b[q]; });
+    return OA_OK;
+  }
+  if (nm == "phi" || nm == "psi" || nm == "psi4") {
+    std::vector<double> c;
+    if (nm == "phi" || !L->phi_valid) {
+      RC(fetch(S_chi, c));
+      if (nm == "phi") emit([&](size_t q, int, int) { return -0.5 * std::log(c[q]); });
+      else if (nm == "psi") emit([&](size_t q, int, int) { return std::exp(-0.5 * std::log(c[q])); });
+      else emit([&](size_t q, int, int) { return 1.0 / (c[q] * c[q]); });
+    } else {
+      RC(fetch(S_phi, c));
+      if (nm == "psi") emit([&](size_t q, int, int) { return std::exp(c[q]); });
+      else emit([&](size_t q, int, int) { const double e = std::exp(c[q]); return e * e * e * e; });
+    }
+    return OA_OK;
+  }
+  if (nm == "KTH" || nm == "lambda" || nm == "Alambda") {
+    std::vector<double> A, B, H, C, KA, KB, KC;
+    RC(fetch(S_A, A)); RC(fetch(S_B, B)); RC(fetch(S_H, H)); RC(fetch(S_C, C));
+    RC(fetch(S_KTA, KA)); RC(fetch(S_KTB, KB)); RC(fetch(S_KTC, KC));
+    emit([&](size_t q, int ii, int) {
+      const double r = (static_cast<double>(ii + 1 - G.g) - 0.5) * G.dr;
+      const double gA = B[q] * H[q], gB = A[q] * H[q], gC = -C[q] * H[q];
+      const double kth = -(gA * KA[q] + gB * KB[q] + 2.0 * r * r * gC * KC[q]) * H[q];
+      if (nm == "KTH") return kth;
+      if (nm == "lambda") return (A[q] - H[q]) / (r * r);
+      return (KA[q] - kth) / (r * r);
+    });
+    return OA_OK;
+  }
+  if (nm == "dtalpha" || nm == "dtphi" || nm == "z4theta") {
+    emit([&](size_t, int, int) { return 0.0; });
+    return OA_OK;
+  }
+  oa_set_error(std::string("oa_download_field: unknown array ") + name);
+  return OA_ERR_ARG;
+}
+
+extern "C" int oa_get_time(const oa_handle* h, int box, int level, double* t, double* t1, double* t2, int* s) {
+  const Level* L = level_of(const_cast<oa_handle*>(h), box, level);
+  if (!L) return OA_ERR_ARG;
+  if (t) *t = L->t;
+  if (t1) *t1 = L->t1;
+  if (t2) *t2 = L->t2;
+  if (s) *s = L->s;
+  return OA_OK;
+}
+
+extern "C" int oa_set_time(oa_handle* h, int box, int level, double t, double t1, double t2, int s) {
+  Level* L = level_of(h, box, level);
+  if (!L) return OA_ERR_ARG;
+  L->t = t; L->t1 = t1; L->t2 = t2; L->s = s;
+  return OA_OK;
+}
+
+extern "C" int oa_saveold(oa_handle* h, int box, int level) {
+  if (!level_of(h, box, level)) return OA_ERR_ARG;
+  return do_saveold(h, level);
+}
+
+extern "C" int oa_substep(oa_handle* h, int box, int level, int iter, int niter, double dtw, double weight) {
+  if (!level_of(h, box, level)) return OA_ERR_ARG;
+  if (iter < 1 || iter > niter) {
+    oa_set_error("oa_substep: iter must lie in 1..niter");
+    return OA_ERR_ARG;
+  }
+  return do_substep(h, level, iter, niter, dtw, weight);
+}
+
+extern "C" int oa_finebound(oa_handle* h, int box, int level, double dtw) {
+  if (!level_of(h, box, level)) return OA_ERR_ARG;
+  return do_finebound(h, level, dtw);
+}
+
+extern "C" int oa_auxiliary(oa_handle* h, int box, int level) {
+  if (!level_of(h, box, level)) return OA_ERR_ARG;
+  return do_auxiliary(h, level);
+}
+
+extern "C" int oa_restrict(oa_handle* h, int box, int level) {
+  if (!level_of(h, box, level)) return OA_ERR_ARG;
+  if (level == 0) {
+    oa_set_error("oa_restrict: level 0 has no coarser level");
+    return OA_ERR_ARG;
+  }
+  return do_restrict(h, level);
+}
+
+extern "C" int oa_advance_time(oa_handle* h, int box, int level) {
+  Level* L = level_of(h, box, level);
+  if (!L) return OA_ERR_ARG;
+  advance_time(*L);
+  return OA_OK;
+}
+
+extern "C" int oa_initial_aux(oa_handle* h) {
+  if (!h) return OA_ERR_ARG;
+  for (int l = 0; l < h->nlev; ++l) {
+    h->L[l].dirty = true;
+    RC(do_auxiliary(h, l));
+  }
+  if (h->p.shift == OA_SHIFT_GD2 && !h->p.driverD0)
+    for (int l = 0; l < h->nlev; ++l)
+      RC(launch_dtbeta_init(h->L[l].G, h->p.drivercsi, h->scalar, h->L[l].slab[h->L[l].cur], h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return OA_OK;
+}
+
+extern "C" int oa_coarse_step(oa_handle* h) {
+  if (!h) return OA_ERR_ARG;
+  HIPCHK(hipEventRecord(h->ev_begin, h->stream));
+  RC(onestep(h, 0));
+  HIPCHK(hipEventRecord(h->ev_end, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev_end));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end));
+  h->last_ms = ms;
+  return OA_OK;
+}
+
+extern "C" int oa_evolve(oa_handle* h, int nsteps) {
+  if (!h || nsteps < 0) return OA_ERR_ARG;
+  for (Level& L : h->L)
+    if (L.dirty) {
+      oa_set_error("oa_evolve: state not projected; call oa_initial_aux after uploading");
+      return OA_ERR_STATE;
+    }
+  h->kev_used = 0;
+  HIPCHK(hipEventRecord(h->ev_begin, h->stream));
+  for (int s = 0; s < nsteps; ++s) RC(onestep(h, 0));
+  HIPCHK(hipEventRecord(h->ev_end, h->stream));
+  HIPCHK(hipEventSynchronize(h->ev_end));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end));
+  h->last_ms = ms;
+  if (h->kernel_timing && h->kev_used > 0) {
+    double sum = 0.0;
+    for (size_t q = 0; q + 1 < h->kev_used; q += 2) {
+      float m = 0.0f;
+      HIPCHK(hipEventElapsedTime(&m, h->kev[q], h->kev[q + 1]));
+      sum += m;
+    }
+    h->rhs_launches = static_cast<long>(h->kev_used / 2);
+    h->rhs_avg_ms = sum / static_cast<double>(h->rhs_launches);
+  }
+  return OA_OK;
+}
+
+extern "C" int oa_sync(oa_handle* h) {
+  if (!h) return OA_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return OA_OK;
+}
+
+extern "C" int oa_last_elapsed_ms(const oa_handle* h, double* ms) {
+  if (!h || !ms) return OA_ERR_ARG;
+  *ms = h->last_ms;
+  return OA_OK;
+}
+
+extern "C" int oa_set_kernel_timing(oa_handle* h, int on) {
+  if (!h) return OA_ERR_ARG;
+  h->kernel_timing = (on != 0);
+  return OA_OK;
+}
+
+extern "C" int oa_rhs_kernel_stats(const oa_handle* h, double* avg_ms, long* launches) {
+  if (!h) return OA_ERR_ARG;
+  if (avg_ms) *avg_ms = h->rhs_avg_ms;
+  if (launches) *launches = h->rhs_launches;
+  return OA_OK;
+}
+
+extern "C" int oa_point_updates(const oa_handle* h, double* updates) {
+  if (!h || !updates) return OA_ERR_ARG;
+  *updates = h->point_updates;
+  return OA_OK;
+}
+
+// accessors used by the host-side initial-data generator
+namespace oax {
+int handle_level_slab(oa_handle* h, int level, double** slab, KGrid* G, int* nf, bool* shift, bool* scalar,
+                      const oa_params** p) {
+  Level& L = h->L[level];
+  *slab = L.slab[L.cur];
+  *G = L.G;
+  *nf = h->nf;
+  *shift = h->shift;
+  *scalar = h->scalar;
+  *p = &h->p;
+  return OA_OK;
+}
+void handle_mark_dirty(oa_handle* h, int level) {
+  h->L[level].dirty = true;
+  h->L[level].phi_valid = true;
+}
+void* handle_stream(oa_handle* h) { return h->stream; }
+}  // namespace oax

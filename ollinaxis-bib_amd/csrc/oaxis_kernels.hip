@@ -1,0 +1,198 @@
+// HIP kernels of the evolution hot path (gfx950, FP64 throughout).
+//
+// One launch of k_rhs performs, for every point of a level, what the
+// reference does in one internal iteration of onestep.f90:129-342:
+//   the derivative stencils and algebra of auxiliary_geometry.f90,
+//   sources_geometry/lapse/shift/scalar, Kreiss-Oliger dissipation,
+//   radiative boundaries on level 0, RK4 accumulate / update,
+//   the axis (and equator) parity images, and the pointwise part of the
+//   next auxiliary call (factordet, phi/chi, KT_H / lambda are implicit).
+// The state stays projected in memory, as in the reference.
+//
+// Layout: a level holds each evolved field as one z-fastest plane; point
+// (i, j) of the reference's (1-g:Nr, 1-g:Nz) range sits at
+// (i+g-1)*ld + (j+g-1).  Ghost columns i <= 0 (axis) and, with equatorial
+// symmetry, ghost rows j <= 0 are stored and kept equal to the parity images
+// of the interior, so every centred stencil reads them directly.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "oa_internal.h"
+
+#define OA_HD __device__ __forceinline__
+#include "oaxis_device.h"
+
+namespace oax {
+
+// ------------------------------------------------------------- RHS kernel
+template <bool SHIFT, bool SCALAR, bool PHIMODE>
+__global__ void __launch_bounds__(256) k_rhs(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
+                                             const double* __restrict__ yp, double* __restrict__ acc,
+                                             double* __restrict__ yout) {
+  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > G.Nz || i > G.Nr) return;
+  rhs_point<SHIFT, SCALAR, PHIMODE>(G, P, S, yin, yp, acc, yout, i, j);
+}
+
+// ------------------------------------------------------- projection pass
+template <bool SHIFT, bool SCALAR>
+__global__ void __launch_bounds__(256) k_project(KGrid G, KPhys P, int time0, double* __restrict__ y) {
+  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > G.Nz || i > G.Nr) return;
+  project_point<SHIFT, SCALAR>(G, P, time0, y, i, j);
+}
+
+__global__ void __launch_bounds__(256) k_deldef(KGrid G, double* __restrict__ y) {
+  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > G.Nz || i > G.Nr) return;
+  deldef_point(G, y, i, j);
+}
+
+// dtbeta = drivercsi * Delta (evolve.f90:174-195), whole arrays incl. images
+__global__ void k_dtbeta_init(long n, double csi, const double* __restrict__ dr, const double* __restrict__ dz,
+                              double* __restrict__ tr, double* __restrict__ tz) {
+  const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  tr[q] = csi * dr[q];
+  tz[q] = csi * dz[q];
+}
+
+// saveold for the fine-boundary strips (one thread per strip element)
+__global__ void k_saveold(KGrid G, int nf, const double* __restrict__ y, double* __restrict__ rR,
+                          double* __restrict__ zL, double* __restrict__ zR) {
+  const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int f = blockIdx.y;
+  if (f >= nf) return;
+  saveold_point(G, y, rR, zL, zR, f, q);
+}
+
+// finebound(box 0, level, dtw, .true.): one thread per fine strip point
+template <bool SHIFT, bool SCALAR>
+__global__ void __launch_bounds__(256) k_finebound(KGrid Gf, KGrid Gc, KPhys P, KTime T, int time0, int interporder,
+                                                   const double* __restrict__ yc, double* __restrict__ yf,
+                                                   const double* __restrict__ rR0, const double* __restrict__ rR1,
+                                                   const double* __restrict__ zL0, const double* __restrict__ zL1,
+                                                   const double* __restrict__ zR0, const double* __restrict__ zR1) {
+  const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  finebound_point<SHIFT, SCALAR>(Gf, Gc, P, T, time0, interporder, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1, q);
+}
+
+// restrict(box 0, level, .true.): one thread per coarse point
+__global__ void __launch_bounds__(256) k_restrict(KGrid Gf, KGrid Gc, int nf, int interporder,
+                                                  const double* __restrict__ yf, double* __restrict__ yc) {
+  const int jc = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + 1 - Gc.g;
+  const int ic = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) + 1 - Gc.g;
+  if (jc > Gc.Nz || ic > Gc.Nr) return;
+  restrict_point(Gf, Gc, nf, interporder, yf, yc, ic, jc);
+}
+
+// ------------------------------------------------------------- launchers
+static dim3 level_grid(const KGrid& G, dim3 blk) {
+  const int nj = G.Nz - G.jlo + 1, ni = G.Nr;
+  return dim3((nj + blk.x - 1) / blk.x, (ni + blk.y - 1) / blk.y, 1);
+}
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    oa_set_error(std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return OA_ERR_HIP;
+  }
+  return OA_OK;
+}
+
+int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar, bool phimode,
+               const double* yin, const double* yp, double* acc, double* yout, void* stream) {
+  const dim3 blk(64, 4, 1);
+  const dim3 grd = level_grid(G, blk);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define LAUNCH_RHS(SH, SC, T)                                                                   \
+  hipLaunchKernelGGL((k_rhs<SH, SC, T>), grd, blk, 0, st, G, P, S, yin, yp, acc, yout)
+  const int code = (shift ? 4 : 0) + (scalar ? 2 : 0) + (phimode ? 1 : 0);
+  switch (code) {
+    case 0: LAUNCH_RHS(false, false, false); break;
+    case 1: LAUNCH_RHS(false, false, true); break;
+    case 2: LAUNCH_RHS(false, true, false); break;
+    case 3: LAUNCH_RHS(false, true, true); break;
+    case 4: LAUNCH_RHS(true, false, false); break;
+    case 5: LAUNCH_RHS(true, false, true); break;
+    case 6: LAUNCH_RHS(true, true, false); break;
+    default: LAUNCH_RHS(true, true, true); break;
+  }
+#undef LAUNCH_RHS
+  return check_launch();
+}
+
+int launch_project(const KGrid& G, const KPhys& P, bool shift, bool scalar, bool time0, double* y, void* stream) {
+  const dim3 blk(64, 4, 1);
+  const dim3 grd = level_grid(G, blk);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int t0 = time0 ? 1 : 0;
+  if (shift && scalar) hipLaunchKernelGGL((k_project<true, true>), grd, blk, 0, st, G, P, t0, y);
+  else if (shift) hipLaunchKernelGGL((k_project<true, false>), grd, blk, 0, st, G, P, t0, y);
+  else if (scalar) hipLaunchKernelGGL((k_project<false, true>), grd, blk, 0, st, G, P, t0, y);
+  else hipLaunchKernelGGL((k_project<false, false>), grd, blk, 0, st, G, P, t0, y);
+  return check_launch();
+}
+
+int launch_deldef(const KGrid& G, double* y, void* stream) {
+  const dim3 blk(64, 4, 1);
+  hipLaunchKernelGGL(k_deldef, level_grid(G, blk), blk, 0, static_cast<hipStream_t>(stream), G, y);
+  return check_launch();
+}
+
+int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream) {
+  (void)scalar;
+  const long n = G.plane;
+  const int blk = 256;
+  const long pl = G.plane;
+  hipLaunchKernelGGL(k_dtbeta_init, dim3(static_cast<unsigned>((n + blk - 1) / blk)), dim3(blk), 0,
+                     static_cast<hipStream_t>(stream), n, csi, y + S_Delta_r * pl, y + S_Delta_z * pl,
+                     y + shift_slot(SH_dtbeta_r) * pl, y + shift_slot(SH_dtbeta_z) * pl);
+  return check_launch();
+}
+
+int launch_saveold(const KGrid& G, int nf, const double* y, double* rR, double* zL, double* zR, void* stream) {
+  const int nrt = G.Nr + G.g, nzt = G.Nz + G.g;
+  const int per = G.g * (nrt > nzt ? nrt : nzt);
+  const int blk = 256;
+  hipLaunchKernelGGL(k_saveold, dim3((per + blk - 1) / blk, nf), dim3(blk), 0, static_cast<hipStream_t>(stream),
+                     G, nf, y, rR, zL, zR);
+  return check_launch();
+}
+
+int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, bool shift, bool scalar,
+                     bool time0, const double* yc, double* yf, const double* rR0, const double* rR1,
+                     const double* zL0, const double* zL1, const double* zR0, const double* zR1, int interporder,
+                     void* stream) {
+  const int g = Gf.g, D = T.depth;
+  const long n_r = static_cast<long>(D + 1) * (Gf.Nz + g);
+  const long n_z = static_cast<long>(D + 1) * (Gf.eqsym ? 1 : 2) * (Gf.Nr + g);
+  const long n = n_r + n_z;
+  const int blk = 256;
+  const dim3 grd(static_cast<unsigned>((n + blk - 1) / blk));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int t0 = time0 ? 1 : 0;
+#define LAUNCH_FB(SH, SC) \
+  hipLaunchKernelGGL((k_finebound<SH, SC>), grd, dim3(blk), 0, st, Gf, Gc, P, T, t0, interporder, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1)
+  if (shift && scalar) LAUNCH_FB(true, true);
+  else if (shift) LAUNCH_FB(true, false);
+  else if (scalar) LAUNCH_FB(false, true);
+  else LAUNCH_FB(false, false);
+#undef LAUNCH_FB
+  return check_launch();
+}
+
+int launch_restrict(const KGrid& Gf, const KGrid& Gc, int nf, int interporder, const double* yf, double* yc,
+                    void* stream) {
+  const dim3 blk(64, 4, 1);
+  const dim3 grd((Gc.Nz + Gc.g + blk.x - 1) / blk.x, (Gc.Nr + Gc.g + blk.y - 1) / blk.y, 1);
+  hipLaunchKernelGGL(k_restrict, grd, blk, 0, static_cast<hipStream_t>(stream), Gf, Gc, nf, interporder, yf, yc);
+  return check_launch();
+}
+
+}  // namespace oax

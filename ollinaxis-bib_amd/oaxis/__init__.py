@@ -1,0 +1,210 @@
+"""Python host side of the MI355X evolution hot path of OllinAxis-BiB.
+
+It mirrors what the reference's Fortran main loop does around the hot path
+(src/base/evolve.f90, onestep.f90): build the parameters from a par file,
+create the device-resident grid hierarchy, set initial data, run the
+Berger-Oliger steps, read arrays back.  All numerical work happens in
+liboaxis.so (HIP, gfx950) behind the C ABI of include/oaxis.h.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import ENUMS, Params, declared_symbols
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI (raises ImportError when liboaxis.so is missing)."""
+    global _lib
+    if _lib is None:
+        _lib = _abi.load_library()
+    return _lib
+
+
+class OaxisError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != _abi.OA_OK:
+        raise OaxisError("%s failed (code %d): %s" % (what, rc, lib().oa_last_error().decode()))
+
+
+def default_params():
+    p = Params()
+    _check(lib().oa_params_default(ctypes.byref(p)), "oa_params_default")
+    return p
+
+
+def params_from_text(text, base=None):
+    """Parameters from par-file text (reference syntax), on top of the
+    reference defaults or of ``base``."""
+    p = default_params() if base is None else _copy(base)
+    _check(lib().oa_params_parse_text(ctypes.byref(p), text.encode()), "par-file parse")
+    return p
+
+
+def params_from_file(path):
+    p = default_params()
+    _check(lib().oa_params_parse_file(ctypes.byref(p), path.encode()), "par-file parse")
+    return p
+
+
+def params_from_dict(d, base=None):
+    p = default_params() if base is None else _copy(base)
+    for k, v in d.items():
+        _check(lib().oa_params_set(ctypes.byref(p), str(k).encode(), str(v).encode()), "set %s" % k)
+    return p
+
+
+def param_text(p, key):
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().oa_params_get(ctypes.byref(p), key.encode(), buf, 256), "get %s" % key)
+    return buf.value.decode()
+
+
+def _copy(p):
+    q = Params()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(Params))
+    return q
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Spacetime:
+    """One device-resident evolution (box 0, all refinement levels).
+
+    Arrays are exchanged in the reference layout: shape (Nr+ghost, Nz+ghost),
+    Fortran order, element [i+ghost-1, j+ghost-1] holding point (i, j).
+    """
+
+    def __init__(self, params):
+        self.params = _copy(params)
+        h = ctypes.c_void_p()
+        _check(lib().oa_create(ctypes.byref(self.params), ctypes.byref(h)), "oa_create")
+        self._h = h
+        n = ctypes.c_int()
+        _check(lib().oa_num_levels(self._h, ctypes.byref(n)), "oa_num_levels")
+        self.nlevels = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().oa_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def level_info(self, level=0):
+        Nr, Nz, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        dr, dz, dt = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().oa_level_info(self._h, 0, level, ctypes.byref(Nr), ctypes.byref(Nz), ctypes.byref(g),
+                                   ctypes.byref(dr), ctypes.byref(dz), ctypes.byref(dt)), "oa_level_info")
+        return dict(Nr=Nr.value, Nz=Nz.value, ghost=g.value, dr=dr.value, dz=dz.value, dt=dt.value)
+
+    def shape(self, level=0):
+        i = self.level_info(level)
+        return (i["Nr"] + i["ghost"], i["Nz"] + i["ghost"])
+
+    def upload(self, name, array, level=0):
+        a = np.asfortranarray(np.asarray(array, dtype=np.float64))
+        if a.shape != self.shape(level):
+            raise ValueError("array %s has shape %s, level %d needs %s" % (name, a.shape, level, self.shape(level)))
+        _check(lib().oa_upload_field(self._h, 0, level, name.encode(), _dptr(a), a.shape[0]), "upload " + name)
+
+    def download(self, name, level=0):
+        a = np.zeros(self.shape(level), dtype=np.float64, order="F")
+        _check(lib().oa_download_field(self._h, 0, level, name.encode(), _dptr(a), a.shape[0]), "download " + name)
+        return a
+
+    def initial_data(self):
+        _check(lib().oa_initial_data(self._h), "oa_initial_data")
+
+    def initial_aux(self):
+        _check(lib().oa_initial_aux(self._h), "oa_initial_aux")
+
+    def coarse_step(self):
+        _check(lib().oa_coarse_step(self._h), "oa_coarse_step")
+
+    def evolve(self, nsteps):
+        _check(lib().oa_evolve(self._h, int(nsteps)), "oa_evolve")
+
+    def sync(self):
+        _check(lib().oa_sync(self._h), "oa_sync")
+
+    def time(self, level=0):
+        t, t1, t2 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        s = ctypes.c_int()
+        _check(lib().oa_get_time(self._h, 0, level, ctypes.byref(t), ctypes.byref(t1), ctypes.byref(t2),
+                                 ctypes.byref(s)), "oa_get_time")
+        return t.value, t1.value, t2.value, s.value
+
+    def set_time(self, level, t, t1, t2, s):
+        _check(lib().oa_set_time(self._h, 0, level, t, t1, t2, s), "oa_set_time")
+
+    # pieces of the Berger-Oliger step, in the reference's call order
+    def saveold(self, level):
+        _check(lib().oa_saveold(self._h, 0, level), "oa_saveold")
+
+    def substep(self, level, it, niter, dtw, w):
+        _check(lib().oa_substep(self._h, 0, level, it, niter, dtw, w), "oa_substep")
+
+    def finebound(self, level, dtw):
+        _check(lib().oa_finebound(self._h, 0, level, dtw), "oa_finebound")
+
+    def auxiliary(self, level):
+        _check(lib().oa_auxiliary(self._h, 0, level), "oa_auxiliary")
+
+    def restrict(self, level):
+        _check(lib().oa_restrict(self._h, 0, level), "oa_restrict")
+
+    def advance_time(self, level):
+        _check(lib().oa_advance_time(self._h, 0, level), "oa_advance_time")
+
+    # timing
+    def last_elapsed_ms(self):
+        v = ctypes.c_double()
+        _check(lib().oa_last_elapsed_ms(self._h, ctypes.byref(v)), "oa_last_elapsed_ms")
+        return v.value
+
+    def set_kernel_timing(self, on):
+        _check(lib().oa_set_kernel_timing(self._h, 1 if on else 0), "oa_set_kernel_timing")
+
+    def rhs_kernel_stats(self):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(lib().oa_rhs_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(n)), "oa_rhs_kernel_stats")
+        return ms.value, n.value
+
+    def point_updates(self):
+        v = ctypes.c_double()
+        _check(lib().oa_point_updates(self._h, ctypes.byref(v)), "oa_point_updates")
+        return v.value
+
+
+# names of the evolved state of a configuration (reference array names)
+BASE_FIELDS = ["alpha", "chi", "phi", "A", "B", "H", "C", "trK", "KTA", "KTB", "KTC", "Delta_r", "Delta_z"]
+SHIFT_FIELDS = ["beta_r", "beta_z", "dtbeta_r", "dtbeta_z"]
+SCALAR_FIELDS = ["scalar_phi", "scalar_xi_r", "scalar_xi_z", "scalar_pi"]
+
+
+def state_fields(params):
+    names = list(BASE_FIELDS)
+    if params.shift != ENUMS["OA_SHIFT_NONE"]:
+        names += SHIFT_FIELDS
+    if params.mattertype & ENUMS["OA_MATTER_SCALAR"]:
+        names += SCALAR_FIELDS
+    return names

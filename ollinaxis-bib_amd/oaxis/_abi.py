@@ -1,0 +1,121 @@
+"""ctypes binding of the C ABI in include/oaxis.h (liboaxis.so).
+
+The library is the HIP implementation; there is no CPU fallback.  Loading
+fails with an ImportError naming the missing file when liboaxis.so has not
+been built (``python -c "import __graft_entry__; __graft_entry__.build()"``).
+"""
+import ctypes
+import os
+import re
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(os.path.dirname(_HERE))
+HEADER = os.path.join(_ROOT, "include", "oaxis.h")
+LIBPATH = os.path.join(_HERE, "liboaxis.so")
+
+OA_OK = 0
+
+
+def _parse_params_struct(path=HEADER):
+    """Field list of ``oa_params`` read from the public header, so the ctypes
+    mirror cannot drift from the C definition."""
+    text = open(path).read()
+    body = text[text.index("typedef struct oa_params {"):text.index("} oa_params;")]
+    fields = []
+    for line in body.split("\n")[1:]:
+        line = line.split("/*")[0].strip()
+        if not line:
+            continue
+        m = re.match(r"(int|double)\s+(.*);", line)
+        if not m:
+            raise RuntimeError("unexpected line in oa_params: %r" % line)
+        ctype = ctypes.c_int if m.group(1) == "int" else ctypes.c_double
+        for name in m.group(2).split(","):
+            fields.append((name.strip(), ctype))
+    return fields
+
+
+def _parse_enums(path=HEADER):
+    text = open(path).read()
+    out = {}
+    for block in re.findall(r"enum\s*{([^}]*)}", text):
+        value = -1
+        for item in block.split(","):
+            item = item.strip()
+            if not item:
+                continue
+            if "=" in item:
+                name, val = [s.strip() for s in item.split("=")]
+                value = int(val)
+            else:
+                name = item
+                value += 1
+            out[name] = value
+    for name, val in re.findall(r"#define\s+(OA_\w+)\s+(\d+)", text):
+        out[name] = int(val)
+    return out
+
+
+class Params(ctypes.Structure):
+    _fields_ = _parse_params_struct()
+
+
+ENUMS = _parse_enums()
+
+
+def load_library(path=LIBPATH):
+    if not os.path.exists(path):
+        raise ImportError(
+            "liboaxis.so not found at %s: build it with __graft_entry__.build() "
+            "(the library is HIP-only; there is no CPU fallback)" % path)
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER(Params)
+    H = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "oa_params_default": (ctypes.c_int, [P]),
+        "oa_params_set": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_char_p]),
+        "oa_params_parse_file": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "oa_params_parse_text": (ctypes.c_int, [P, ctypes.c_char_p]),
+        "oa_params_get": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
+        "oa_params_check_supported": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int]),
+        "oa_params_size": (ctypes.c_size_t, []),
+        "oa_last_error": (ctypes.c_char_p, []),
+        "oa_create": (ctypes.c_int, [P, ctypes.POINTER(H)]),
+        "oa_destroy": (ctypes.c_int, [H]),
+        "oa_num_levels": (ctypes.c_int, [H, ip]),
+        "oa_level_info": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ip, ip, ip, dp, dp, dp]),
+        "oa_upload_field": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, ctypes.c_int]),
+        "oa_download_field": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp, ctypes.c_int]),
+        "oa_get_time": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, dp, dp, dp, ip]),
+        "oa_set_time": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int]),
+        "oa_saveold": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
+        "oa_substep": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double]),
+        "oa_finebound": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+        "oa_auxiliary": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
+        "oa_restrict": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
+        "oa_advance_time": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
+        "oa_initial_aux": (ctypes.c_int, [H]),
+        "oa_coarse_step": (ctypes.c_int, [H]),
+        "oa_evolve": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_sync": (ctypes.c_int, [H]),
+        "oa_initial_data": (ctypes.c_int, [H]),
+        "oa_last_elapsed_ms": (ctypes.c_int, [H, dp]),
+        "oa_set_kernel_timing": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_rhs_kernel_stats": (ctypes.c_int, [H, dp, ctypes.POINTER(ctypes.c_long)]),
+        "oa_point_updates": (ctypes.c_int, [H, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def declared_symbols(path=HEADER):
+    """Names of every function the public header declares."""
+    text = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int|size_t)\s+(oa_\w+)\s*\(", text, re.M)))
