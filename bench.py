@@ -91,12 +91,6 @@ def main():
     scalar = bool(params.mattertype & oaxis.ENUMS["OA_MATTER_SCALAR"])
     nfields = workloads.evolved_count(shift, scalar)
 
-    try:
-        import torch
-        have_torch = torch.cuda.is_available()
-    except Exception:
-        have_torch = False
-
     st = oaxis.Spacetime(params)
     st.initial_data()
     st.initial_aux()
@@ -104,14 +98,10 @@ def main():
         st.evolve(args.warmup)
     u0 = st.point_updates()
     st.set_kernel_timing(True)
-    if have_torch:
-        torch.cuda.synchronize()
     st.sync()
     t0 = time.perf_counter()
     st.evolve(args.steps)
     st.sync()
-    if have_torch:
-        torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     updates = st.point_updates() - u0
     rhs_ms, rhs_n = st.rhs_kernel_stats()

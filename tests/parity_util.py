@@ -12,6 +12,8 @@ TOL = 1.0e-10      # relative max-norm agreement required for FP64 parity
 
 
 def rel_err(a, b):
+    if not np.all(np.isfinite(a)):
+        return float("inf")
     scale = np.abs(b).max()
     diff = np.abs(a - b).max()
     if scale == 0.0:
@@ -37,7 +39,9 @@ def compare_state(oaxis, st, orc, params, extra=("KTH", "lambda", "Alambda")):
         for name in list(oaxis.state_fields(params)) + list(extra):
             a = st.download(name, lev)
             b = orc.get(name, lev)
-            if name in R2_SCALED:
+            if not np.all(np.isfinite(a)):
+                out[(lev, name)] = float("inf")
+            elif name in R2_SCALED:
                 u, v = R2_SCALED[name]
                 scale = max(np.abs(orc.get(u, lev)).max(), np.abs(orc.get(v, lev)).max())
                 out[(lev, name)] = np.abs((a - b) * r * r).max() / scale if scale > 0 else 0.0
