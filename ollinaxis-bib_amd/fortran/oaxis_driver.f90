@@ -1,0 +1,51 @@
+! Minimal Fortran main loop over liboaxis.so: what main.f90 + evolve.f90 of
+! the reference do for the evolution, with the substep body replaced by the
+! C ABI.  Usage:  oaxis_driver <parfile> <coarse steps> <output file>
+! Writes alpha, chi and trK of level 0 as raw doubles in the reference
+! layout (1-ghost:Nr, 1-ghost:Nz), column-major, and prints a summary line.
+program oaxis_driver
+  use, intrinsic :: iso_c_binding
+  use oaxis_mod
+  implicit none
+  type(oaxis_params), target :: par
+  type(c_ptr) :: h
+  character(len=1024) :: parfile, arg, outfile
+  integer(c_int) :: nsteps, Nr, Nz, ghost, nlev
+  real(c_double) :: dr, dz, dt, t, t1, t2, ms
+  integer(c_int) :: s
+  real(c_double), allocatable, target :: alpha(:,:), chi(:,:), trK(:,:)
+  character(kind=c_char) :: msg(512)
+  integer :: u
+
+  call get_command_argument(1, parfile)
+  call get_command_argument(2, arg)
+  read (arg, *) nsteps
+  call get_command_argument(3, outfile)
+
+  call oaxis_params_init(par)
+  call oaxis_check(oa_params_parse_file(c_loc(par%bytes), c_str(parfile)), "oa_params_parse_file")
+  call oaxis_check(oa_params_check_supported(c_loc(par%bytes), msg, 512_c_int), "oa_params_check_supported")
+
+  h = c_null_ptr
+  call oaxis_check(oa_create(c_loc(par%bytes), h), "oa_create")
+  call oaxis_check(oa_num_levels(h, nlev), "oa_num_levels")
+  call oaxis_check(oa_level_info(h, 0_c_int, 0_c_int, Nr, Nz, ghost, dr, dz, dt), "oa_level_info")
+
+  call oaxis_check(oa_initial_data(h), "oa_initial_data")
+  call oaxis_check(oa_initial_aux(h), "oa_initial_aux")
+  call oaxis_check(oa_evolve(h, nsteps), "oa_evolve")
+  call oaxis_check(oa_last_elapsed_ms(h, ms), "oa_last_elapsed_ms")
+  call oaxis_check(oa_get_time(h, 0_c_int, 0_c_int, t, t1, t2, s), "oa_get_time")
+
+  allocate(alpha(1-ghost:Nr, 1-ghost:Nz), chi(1-ghost:Nr, 1-ghost:Nz), trK(1-ghost:Nr, 1-ghost:Nz))
+  call oaxis_check(oa_download_field(h, 0_c_int, 0_c_int, c_str("alpha"), alpha, Nr + ghost), "download alpha")
+  call oaxis_check(oa_download_field(h, 0_c_int, 0_c_int, c_str("chi"), chi, Nr + ghost), "download chi")
+  call oaxis_check(oa_download_field(h, 0_c_int, 0_c_int, c_str("trK"), trK, Nr + ghost), "download trK")
+
+  open(newunit=u, file=trim(outfile), access="stream", form="unformatted", status="replace")
+  write (u) alpha, chi, trK
+  close(u)
+  print '(a,i0,a,i0,a,i0,a,i0,a,es24.16,a,es24.16,a,f10.3)', 'levels ', nlev, ' Nr ', Nr, ' Nz ', Nz, &
+      ' steps ', s, ' t ', t, ' min(alpha) ', minval(alpha), ' ms ', ms
+  call oaxis_check(oa_destroy(h), "oa_destroy")
+end program oaxis_driver

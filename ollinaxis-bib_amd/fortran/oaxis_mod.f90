@@ -1,0 +1,225 @@
+! ISO_C_BINDING interface to liboaxis.so, the MI355X evolution path of
+! OllinAxis-BiB.  A Fortran main loop (main.f90 / evolve.f90 / onestep.f90 of
+! the reference) calls these in place of the routines each one names.
+!
+! oa_params is handled as an opaque buffer of oa_params_size() bytes: it is
+! filled from the reference's own par file (oa_params_parse_file) or key by
+! key (oa_params_set), so the Fortran side never mirrors its layout.
+! Every function returns 0 (OA_OK) on success; oaxis_check turns a nonzero
+! code into the reference's fatal-error behaviour (print + stop, die.f90).
+module oaxis_mod
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: OA_OK = 0
+
+  public :: oaxis_params, oaxis_params_init, oaxis_check, c_str
+  public :: oa_params_default, oa_params_set, oa_params_parse_file, oa_params_get, &
+            oa_params_check_supported, oa_params_size, oa_last_error, &
+            oa_create, oa_destroy, oa_num_levels, oa_level_info, &
+            oa_upload_field, oa_download_field, oa_get_time, oa_set_time, &
+            oa_saveold, oa_substep, oa_finebound, oa_auxiliary, oa_restrict, oa_advance_time, &
+            oa_initial_aux, oa_coarse_step, oa_evolve, oa_sync, oa_initial_data, &
+            oa_last_elapsed_ms, oa_point_updates
+
+  ! storage for one oa_params block
+  type :: oaxis_params
+    integer(c_int8_t), allocatable :: bytes(:)
+  end type oaxis_params
+
+  interface
+    ! include/oaxis.h: parameter block (param.f90, parse.f90)
+    integer(c_int) function oa_params_default(p) bind(C, name="oa_params_default")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+    end function
+    integer(c_int) function oa_params_set(p, key, val) bind(C, name="oa_params_set")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: p
+      character(kind=c_char), dimension(*) :: key, val
+    end function
+    integer(c_int) function oa_params_parse_file(p, path) bind(C, name="oa_params_parse_file")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: p
+      character(kind=c_char), dimension(*) :: path
+    end function
+    integer(c_int) function oa_params_get(p, key, out, outlen) bind(C, name="oa_params_get")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: p
+      character(kind=c_char), dimension(*) :: key
+      character(kind=c_char), dimension(*) :: out
+      integer(c_int), value :: outlen
+    end function
+    integer(c_int) function oa_params_check_supported(p, msg, msglen) bind(C, name="oa_params_check_supported")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: p
+      character(kind=c_char), dimension(*) :: msg
+      integer(c_int), value :: msglen
+    end function
+    integer(c_size_t) function oa_params_size() bind(C, name="oa_params_size")
+      import :: c_size_t
+    end function
+    type(c_ptr) function oa_last_error() bind(C, name="oa_last_error")
+      import :: c_ptr
+    end function
+
+    ! handles (main.f90 allocatearrays / setupboxes.f90)
+    integer(c_int) function oa_create(p, h) bind(C, name="oa_create")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+      type(c_ptr) :: h
+    end function
+    integer(c_int) function oa_destroy(h) bind(C, name="oa_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function oa_num_levels(h, n) bind(C, name="oa_num_levels")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int) :: n
+    end function
+    integer(c_int) function oa_level_info(h, box, level, Nr, Nz, ghost, dr, dz, dt) &
+        bind(C, name="oa_level_info")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      integer(c_int) :: Nr, Nz, ghost
+      real(c_double) :: dr, dz, dt
+    end function
+
+    ! grid(box,level)%var <-> device, reference layout (1-ghost:Nr, 1-ghost:Nz)
+    integer(c_int) function oa_upload_field(h, box, level, name, src, ld) bind(C, name="oa_upload_field")
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      character(kind=c_char), dimension(*) :: name
+      real(c_double), dimension(*) :: src
+      integer(c_int), value :: ld
+    end function
+    integer(c_int) function oa_download_field(h, box, level, name, dst, ld) bind(C, name="oa_download_field")
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      character(kind=c_char), dimension(*) :: name
+      real(c_double), dimension(*) :: dst
+      integer(c_int), value :: ld
+    end function
+    integer(c_int) function oa_get_time(h, box, level, t, t1, t2, s) bind(C, name="oa_get_time")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      real(c_double) :: t, t1, t2
+      integer(c_int) :: s
+    end function
+    integer(c_int) function oa_set_time(h, box, level, t, t1, t2, s) bind(C, name="oa_set_time")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      real(c_double), value :: t, t1, t2
+      integer(c_int), value :: s
+    end function
+
+    ! the substep body of onestep.f90, callable piecewise
+    integer(c_int) function oa_saveold(h, box, level) bind(C, name="oa_saveold")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+    end function
+    integer(c_int) function oa_substep(h, box, level, iter, niter, dtw, w) bind(C, name="oa_substep")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level, iter, niter
+      real(c_double), value :: dtw, w
+    end function
+    integer(c_int) function oa_finebound(h, box, level, dtw) bind(C, name="oa_finebound")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      real(c_double), value :: dtw
+    end function
+    integer(c_int) function oa_auxiliary(h, box, level) bind(C, name="oa_auxiliary")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+    end function
+    integer(c_int) function oa_restrict(h, box, level) bind(C, name="oa_restrict")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+    end function
+    integer(c_int) function oa_advance_time(h, box, level) bind(C, name="oa_advance_time")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+    end function
+
+    ! whole steps (evolve.f90 / onestep(0))
+    integer(c_int) function oa_initial_data(h) bind(C, name="oa_initial_data")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function oa_initial_aux(h) bind(C, name="oa_initial_aux")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function oa_coarse_step(h) bind(C, name="oa_coarse_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function oa_evolve(h, n) bind(C, name="oa_evolve")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: n
+    end function
+    integer(c_int) function oa_sync(h) bind(C, name="oa_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function oa_last_elapsed_ms(h, ms) bind(C, name="oa_last_elapsed_ms")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: ms
+    end function
+    integer(c_int) function oa_point_updates(h, u) bind(C, name="oa_point_updates")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: u
+    end function
+  end interface
+
+contains
+
+  ! Fortran string -> NUL-terminated C string
+  function c_str(s) result(c)
+    character(len=*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s) + 1) :: c
+    c = trim(s) // c_null_char
+  end function
+
+  ! allocate a parameter block and fill it with the reference defaults
+  subroutine oaxis_params_init(p)
+    type(oaxis_params), target, intent(inout) :: p
+    if (allocated(p%bytes)) deallocate(p%bytes)
+    allocate(p%bytes(oa_params_size()))
+    call oaxis_check(oa_params_default(c_loc(p%bytes)), "oa_params_default")
+  end subroutine
+
+  ! nonzero return code -> message and stop, as die.f90 does
+  subroutine oaxis_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == OA_OK) return
+    call c_f_pointer(oa_last_error(), msg, [4096])
+    n = 0
+    do while (n < 4096)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    print *, trim(what), ' failed (code ', rc, '): ', transfer(msg(1:n), repeat(' ', n))
+    stop 1
+  end subroutine
+
+end module oaxis_mod
