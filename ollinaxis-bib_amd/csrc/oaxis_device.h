@@ -964,7 +964,7 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
                            int interporder, const double* __restrict__ yc, double* __restrict__ yf,
                            const double* __restrict__ rR0, const double* __restrict__ rR1,
                            const double* __restrict__ zL0, const double* __restrict__ zL1,
-                           const double* __restrict__ zR0, const double* __restrict__ zR1, long q) {
+                           const double* __restrict__ zR0, const double* __restrict__ zR1, long q, int grp) {
   const int g = Gf.g, D = T.depth;
   const int nrt = Gf.Nr + g, nzt = Gf.Nz + g;
   const int nz_strips = Gf.eqsym ? 1 : 2;
@@ -1008,36 +1008,42 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
   else { s0 = zL0; s1 = zL1; sidx = static_cast<long>(n) * nrt + (i + g - 1); sper = static_cast<long>(g) * nrt; }
   const bool strip_ok = (which == 0 ? m : n) < g;
 
-  constexpr int ns = nslots(SHIFT, SCALAR);
-  double v[ns];
-  for (int s = 0; s < ns; ++s) {
+  // time interpolation of one injected value (bound_interp.inc)
+  auto value = [&](int s) -> double {
     const double aux2 = apply(c, yc + s * pc, Gc);
     const double b0 = strip_ok ? s0[s * sper + sidx] : 0.0;   // state at t0 (var_p)
-    double val;
-    if (T.border == 1) {
-      val = T.c_aux * aux2 + (T.same ? 0.0 : T.c_p * b0);
-    } else {
-      const double b1 = strip_ok ? s1[s * sper + sidx] : 0.0;
-      val = T.c0 * aux2 + (T.same ? 0.0 : T.c1 * b0 + T.c2 * b1);
-    }
-    v[s] = val;
-  }
-  const double r = coord_r(Gf, i);
-  Proj qp{v[S_chi], v[S_phi], v[S_A], v[S_B], v[S_H], v[S_C]};
-  project(qp, r, time0 != 0, P.chimethod, SCALAR);
-  v[S_chi] = qp.chi; v[S_phi] = qp.phi; v[S_A] = qp.A; v[S_B] = qp.B; v[S_H] = qp.H; v[S_C] = qp.C;
-  for (int s = 0; s < ns; ++s) {
+    if (T.border == 1) return T.c_aux * aux2 + (T.same ? 0.0 : T.c_p * b0);
+    const double b1 = strip_ok ? s1[s * sper + sidx] : 0.0;
+    return T.c0 * aux2 + (T.same ? 0.0 : T.c1 * b0 + T.c2 * b1);
+  };
+  if (grp == 0) {
+    // the fields coupled by the projection (chi/phi, factordet) go together
+    const double r = coord_r(Gf, i);
+    Proj qp{value(S_chi), value(S_phi), value(S_A), value(S_B), value(S_H), value(S_C)};
+    project(qp, r, time0 != 0, P.chimethod, SCALAR);
+    put(yf + S_chi * pf, Gf, i, j, qp.chi, 1.0, 1.0);
+    if (time0 || !P.chimethod) put(yf + S_phi * pf, Gf, i, j, qp.phi, 1.0, 1.0);   // phi is not kept after the first step
+    put(yf + S_A * pf, Gf, i, j, qp.A, 1.0, 1.0);
+    put(yf + S_B * pf, Gf, i, j, qp.B, 1.0, 1.0);
+    put(yf + S_H * pf, Gf, i, j, qp.H, 1.0, 1.0);
+    put(yf + S_C * pf, Gf, i, j, qp.C, 1.0, -1.0);
+  } else {
+    // every other field on its own: alpha, then the slots after S_C
+    const int s = (grp == 1) ? S_alpha : S_C + grp - 1;
     double symr, symz;
     slot_parity(SHIFT, SCALAR, s, symr, symz);
-    if (s == S_phi && !time0 && P.chimethod) continue;   // phi is not kept after the first step
-    put(yf + s * pf, Gf, i, j, v[s], symr, symz);
+    put(yf + s * pf, Gf, i, j, value(s), symr, symz);
   }
   (void)k;
 }
 
+// groups of finebound_point: the projected set, then one per remaining slot
+constexpr int finebound_groups(bool shift, bool scalar) { return 2 + nslots(shift, scalar) - (S_C + 1); }
+
 // restrict(box 0, level, .true.): coarse points well inside the fine level
-// take the bicubic interpolant of the fine level (restrict.f90:88-162).
-OA_HD void restrict_point(const KGrid& Gf, const KGrid& Gc, int nf, int interporder,
+// take the bicubic interpolant of the fine level (restrict.f90:88-162);
+// one call per (coarse point, field slot s).
+OA_HD void restrict_point(const KGrid& Gf, const KGrid& Gc, int s, int interporder,
                           const double* __restrict__ yf, double* __restrict__ yc, int ic, int jc) {
   const int g = Gc.g;
   const double r0 = coord_r(Gc, 1 - g) + static_cast<double>(ic + g - 1) * Gc.dr;
@@ -1052,7 +1058,7 @@ OA_HD void restrict_point(const KGrid& Gf, const KGrid& Gc, int nf, int interpor
   if (!(r0 > rmin && r0 < rmax && z0 > zmin && z0 < zmax)) return;
   const Cell c = locate(Gf, r0, z0, interporder);
   const long k = IDX(Gc, ic, jc);
-  for (int s = 0; s < nf; ++s) yc[s * Gc.plane + k] = apply(c, yf + s * Gf.plane, Gf);
+  yc[s * Gc.plane + k] = apply(c, yf + s * Gf.plane, Gf);
 }
 
 }  // namespace oax

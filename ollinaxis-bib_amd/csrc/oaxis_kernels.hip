@@ -78,16 +78,17 @@ __global__ void __launch_bounds__(256) k_finebound(KGrid Gf, KGrid Gc, KPhys P, 
                                                    const double* __restrict__ zL0, const double* __restrict__ zL1,
                                                    const double* __restrict__ zR0, const double* __restrict__ zR1) {
   const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  finebound_point<SHIFT, SCALAR>(Gf, Gc, P, T, time0, interporder, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1, q);
+  finebound_point<SHIFT, SCALAR>(Gf, Gc, P, T, time0, interporder, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1, q,
+                                 static_cast<int>(blockIdx.y));
 }
 
-// restrict(box 0, level, .true.): one thread per coarse point
-__global__ void __launch_bounds__(256) k_restrict(KGrid Gf, KGrid Gc, int nf, int interporder,
+// restrict(box 0, level, .true.): one thread per (coarse point, field)
+__global__ void __launch_bounds__(256) k_restrict(KGrid Gf, KGrid Gc, int interporder,
                                                   const double* __restrict__ yf, double* __restrict__ yc) {
   const int jc = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + 1 - Gc.g;
   const int ic = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) + 1 - Gc.g;
   if (jc > Gc.Nz || ic > Gc.Nr) return;
-  restrict_point(Gf, Gc, nf, interporder, yf, yc, ic, jc);
+  restrict_point(Gf, Gc, static_cast<int>(blockIdx.z), interporder, yf, yc, ic, jc);
 }
 
 // ------------------------------------------------------------- launchers
@@ -174,7 +175,8 @@ int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTi
   const long n_z = static_cast<long>(D + 1) * (Gf.eqsym ? 1 : 2) * (Gf.Nr + g);
   const long n = n_r + n_z;
   const int blk = 256;
-  const dim3 grd(static_cast<unsigned>((n + blk - 1) / blk));
+  const dim3 grd(static_cast<unsigned>((n + blk - 1) / blk),
+                 static_cast<unsigned>(finebound_groups(shift, scalar)), 1);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int t0 = time0 ? 1 : 0;
 #define LAUNCH_FB(SH, SC) \
@@ -190,8 +192,9 @@ int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTi
 int launch_restrict(const KGrid& Gf, const KGrid& Gc, int nf, int interporder, const double* yf, double* yc,
                     void* stream) {
   const dim3 blk(64, 4, 1);
-  const dim3 grd((Gc.Nz + Gc.g + blk.x - 1) / blk.x, (Gc.Nr + Gc.g + blk.y - 1) / blk.y, 1);
-  hipLaunchKernelGGL(k_restrict, grd, blk, 0, static_cast<hipStream_t>(stream), Gf, Gc, nf, interporder, yf, yc);
+  const dim3 grd((Gc.Nz + Gc.g + blk.x - 1) / blk.x, (Gc.Nr + Gc.g + blk.y - 1) / blk.y,
+                 static_cast<unsigned>(nf));
+  hipLaunchKernelGGL(k_restrict, grd, blk, 0, static_cast<hipStream_t>(stream), Gf, Gc, interporder, yf, yc);
   return check_launch();
 }
 

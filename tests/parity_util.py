@@ -30,6 +30,10 @@ def make_params(oaxis, text):
 # differences the 1/r^2 amplifies near the axis.  They are compared through
 # r^2*lambda, r^2*Alambda, i.e. against the scale of A-H and KTA-KT_H.
 R2_SCALED = {"lambda": ("A", "H"), "Alambda": ("KTA", "KTH")}
+# phi = -log(chi)/2 is a logarithm: its absolute error is the relative error
+# of chi, so it is compared against max(1, max|phi|) rather than against
+# max|phi| alone, which is ~1e-6 for weak scalar pulses.
+LOG_SCALED = ("phi",)
 
 
 def compare_state(oaxis, st, orc, params, extra=("KTH", "lambda", "Alambda")):
@@ -41,6 +45,8 @@ def compare_state(oaxis, st, orc, params, extra=("KTH", "lambda", "Alambda")):
             b = orc.get(name, lev)
             if not np.all(np.isfinite(a)):
                 out[(lev, name)] = float("inf")
+            elif name in LOG_SCALED:
+                out[(lev, name)] = np.abs(a - b).max() / max(1.0, np.abs(b).max())
             elif name in R2_SCALED:
                 u, v = R2_SCALED[name]
                 scale = max(np.abs(orc.get(u, lev)).max(), np.abs(orc.get(v, lev)).max())
