@@ -184,6 +184,28 @@ struct Plane {
   OA_HD double operator()(int i, int j) const { return p[IDX(G, i, j)]; }
 };
 
+// Field views of the right-hand side: every stencil of rhs_point reads its
+// fields through V.fld(slot)(i, j).  GlobalView reads the level slab in HBM;
+// TileView reads a block's tile (with a 3-point halo) staged in LDS, laid out
+// row-major as (i - ib) * tw + (j - jb), one tile of ts values per field.
+struct GlobalView {
+  const double* __restrict__ y;
+  KGrid G;
+  OA_HD Plane fld(int s) const { return Plane{y + s * G.plane, G}; }
+};
+
+struct TilePlane {
+  const double* p;
+  int ib, jb, tw;
+  OA_HD double operator()(int i, int j) const { return p[(i - ib) * tw + (j - jb)]; }
+};
+
+struct TileView {
+  const double* t;
+  int ib, jb, tw, ts;
+  OA_HD TilePlane fld(int s) const { return TilePlane{t + s * ts, ib, jb, tw}; }
+};
+
 struct D1 { double r, z; };
 struct D2 { double r, z, rr, zz, rz; };
 
@@ -251,13 +273,13 @@ OA_HD void slot_parity(bool shift, bool scalar, int s, double& symr, double& sym
 // never formed (they keep their initial zero).  This holds for the first step
 // of a level and for the first internal iteration of its second step.
 // S.proj_t0: the auxiliary call that ends this iteration runs at clock 0.
-template <bool SHIFT, bool SCALAR, bool PHIMODE>
-OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const double* __restrict__ yin,
+template <bool SHIFT, bool SCALAR, bool PHIMODE, class View>
+OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View& V,
                      const double* __restrict__ yp, double* __restrict__ acc, double* __restrict__ yout,
                      int i, int j) {
 
   const long pl = G.plane;
-  auto fld = [&](int s) { return Plane{yin + s * pl, G}; };
+  auto fld = [&](int s) { return V.fld(s); };
   const long k = IDX(G, i, j);
 
   // coordinates (evolve.f90:52-71)
@@ -266,10 +288,10 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const doubl
   const double ir = 1.0 / r;
   const double r2 = r * r;
 
-  const Plane Fa = fld(S_alpha), Fchi = fld(S_chi), Fphi = fld(S_phi);
-  const Plane FA = fld(S_A), FB = fld(S_B), FH = fld(S_H), FC = fld(S_C);
-  const Plane FK = fld(S_trK), FKA = fld(S_KTA), FKB = fld(S_KTB), FKC = fld(S_KTC);
-  const Plane FDr = fld(S_Delta_r), FDz = fld(S_Delta_z);
+  const auto Fa = fld(S_alpha), Fchi = fld(S_chi), Fphi = fld(S_phi);
+  const auto FA = fld(S_A), FB = fld(S_B), FH = fld(S_H), FC = fld(S_C);
+  const auto FK = fld(S_trK), FKA = fld(S_KTA), FKB = fld(S_KTB), FKC = fld(S_KTC);
+  const auto FDr = fld(S_Delta_r), FDz = fld(S_Delta_z);
 
   const double alpha = Fa(i, j), chi = Fchi(i, j);
   const double A = FA(i, j), B = FB(i, j), H = FH(i, j), C = FC(i, j);
@@ -364,7 +386,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const doubl
   D2 dbr_{}, dbz_{};
   double DD_beta_rr = 0.0, DIV_beta = 0.0, Dr_DIV_beta = 0.0, Dz_DIV_beta = 0.0;
   if constexpr (SHIFT) {
-    const Plane Fbr = fld(shift_slot(SH_beta_r)), Fbz = fld(shift_slot(SH_beta_z));
+    const auto Fbr = fld(shift_slot(SH_beta_r)), Fbz = fld(shift_slot(SH_beta_z));
     dbr_ = hess(Fbr, G, i, j);
     dbz_ = hess(Fbz, G, i, j);
     auto fbr_r = [&](int ii, int jj) { return Fbr(ii, jj) / ((static_cast<double>(ii) - 0.5) * G.dr); };
@@ -535,8 +557,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const doubl
   constexpr double smallpi = 3.14159265358979323846;
   const double dscale_s = P.scalardiss / G.dt;
   if constexpr (SCALAR) {
-    const Plane Fsp = fld(scalar_slot(SHIFT, SC_phi)), Fxr = fld(scalar_slot(SHIFT, SC_xi_r));
-    const Plane Fxz = fld(scalar_slot(SHIFT, SC_xi_z)), Fpi = fld(scalar_slot(SHIFT, SC_pi));
+    const auto Fsp = fld(scalar_slot(SHIFT, SC_phi)), Fxr = fld(scalar_slot(SHIFT, SC_xi_r));
+    const auto Fxz = fld(scalar_slot(SHIFT, SC_xi_z)), Fpi = fld(scalar_slot(SHIFT, SC_pi));
     sc_phi = Fsp(i, j); sc_xr = Fxr(i, j); sc_xz = Fxz(i, j); sc_pi = Fpi(i, j);
     double V = 0.0, VP = 0.0;
     if (P.scalarpotential == OA_POT_PHI2) {
@@ -713,7 +735,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const doubl
   // ---- sources_shift (Gammadriver2)
   double sbr = 0.0, sbz = 0.0, sdbr = 0.0, sdbz = 0.0;
   if constexpr (SHIFT) {
-    const Plane Fdbr = fld(shift_slot(SH_dtbeta_r)), Fdbz = fld(shift_slot(SH_dtbeta_z));
+    const auto Fdbr = fld(shift_slot(SH_dtbeta_r)), Fdbz = fld(shift_slot(SH_dtbeta_z));
     sbr = dbr;
     sbz = dbz;
     // the advective derivatives of sources_shift.f90:229-280 are the DA*
@@ -775,7 +797,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const doubl
       if (!last) a[k] = first ? S.w * s : a[k] + S.w * s;
       else sv = a[k] + S.w * s;
     }
-    const double base = first ? yin[slot * pl + k] : yp[slot * pl + k];
+    const double base = first ? fld(slot)(i, j) : yp[slot * pl + k];
     return base + S.dtw * sv;
   };
 
