@@ -20,6 +20,7 @@ enum Slot {
   NBASE
 };
 enum ShiftSlot { SH_beta_r = 0, SH_beta_z, SH_dtbeta_r, SH_dtbeta_z, NSHIFT };
+static_assert(SH_dtbeta_z == SH_dtbeta_r + 1, "dtbeta_r and dtbeta_z planes are adjacent");
 enum ScalarSlot { SC_phi = 0, SC_xi_r, SC_xi_z, SC_pi, NSCALAR };
 
 constexpr int nslots(bool shift, bool scalar) {
@@ -58,6 +59,12 @@ struct KStep {
   int iter, niter, rk4;
   double dtw, w;
   int proj_t0;          // the auxiliary call ending this iteration runs at clock 0
+  // Non-null on a level's first substep after the initial auxiliary pass:
+  // the dtbeta_r, dtbeta_z planes as they were before the Gamma-driver
+  // initialisation dtbeta = drivercsi * Delta.  The reference differentiates
+  // dtbeta in that auxiliary pass, before the initialisation, so the first
+  // substep's dtbeta derivatives come from these planes.
+  const double* dtb_pre;
 };
 
 // Time-interpolation coefficients of one finebound call (computed on the

@@ -733,9 +733,13 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   }
 
   // ---- sources_shift (Gammadriver2)
+  // the planes dtbeta is differentiated from (see KStep::dtb_pre)
+  const double* const dpre = S.dtb_pre;
+  const auto Fdbr = fld(SHIFT ? shift_slot(SH_dtbeta_r) : 0), Fdbz = fld(SHIFT ? shift_slot(SH_dtbeta_z) : 0);
+  auto Ddbr = [&](int ii, int jj) { return dpre ? dpre[IDX(G, ii, jj)] : Fdbr(ii, jj); };
+  auto Ddbz = [&](int ii, int jj) { return dpre ? dpre[pl + IDX(G, ii, jj)] : Fdbz(ii, jj); };
   double sbr = 0.0, sbz = 0.0, sdbr = 0.0, sdbz = 0.0;
   if constexpr (SHIFT) {
-    const auto Fdbr = fld(shift_slot(SH_dtbeta_r)), Fdbz = fld(shift_slot(SH_dtbeta_z));
     sbr = dbr;
     sbz = dbz;
     // the advective derivatives of sources_shift.f90:229-280 are the DA*
@@ -751,10 +755,10 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     sdbz = P.drivercsi * sDz;
     if (P.driveradv) {
       double tr_, tz_, dr_, dz_;
-      adv(Fdbr, grad(Fdbr, G, i, j), tr_, tz_);
+      adv(Ddbr, grad(Ddbr, G, i, j), tr_, tz_);
       adv(FDr, dDr, dr_, dz_);
       sdbr += br * (tr_ - dr_) + bz * (tz_ - dz_);
-      adv(Fdbz, grad(Fdbz, G, i, j), tr_, tz_);
+      adv(Ddbz, grad(Ddbz, G, i, j), tr_, tz_);
       adv(FDz, dDz, dr_, dz_);
       sdbz += br * (tr_ - dr_) + bz * (tz_ - dz_);
     }
@@ -779,8 +783,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
       if constexpr (SHIFT) {
         if (P.drivercsi != 0.0) {
           const double vs = sqrt(4.0 * third * P.drivercsi);
-          const D1 ddr = grad(fld(shift_slot(SH_dtbeta_r)), G, i, j);
-          const D1 ddz = grad(fld(shift_slot(SH_dtbeta_z)), G, i, j);
+          const D1 ddr = grad(Ddbr, G, i, j);
+          const D1 ddz = grad(Ddbz, G, i, j);
           sdbr = -vs * (r * ddr.r + z * ddr.z + dbr) * irr;
           sdbz = -vs * (r * ddz.r + z * ddz.z + dbz) * irr;
         }

@@ -32,6 +32,8 @@ struct Level {
   bool dirty = true;                                // state needs the full projection
   bool phi_valid = true;                            // phi plane holds the reference's phi
   bool aux_phi = true;                              // last auxiliary ran at clock 0 (phi mode)
+  double* dtb_pre = nullptr;                        // dtbeta_r, dtbeta_z before the driver initialisation
+  bool dtb_pre_valid = false;                       // the next substep differentiates dtb_pre
   double t = 0.0, t1 = 0.0, t2 = 0.0;
   int s = 0;
   long strip_r = 0, strip_z = 0;                    // doubles per strip set piece
@@ -182,7 +184,8 @@ int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double 
   const bool phimode = L.aux_phi;
   KPhys P = h->phys;
   P.radiative = (level == 0 && h->p.boundtype == OA_BOUND_RADIATIVE) ? 1 : 0;
-  KStep S{iter, niter, h->p.integrator == OA_RK4 ? 1 : 0, dtw, w, time0 ? 1 : 0};
+  KStep S{iter, niter, h->p.integrator == OA_RK4 ? 1 : 0, dtw, w, time0 ? 1 : 0,
+          L.dtb_pre_valid ? L.dtb_pre : nullptr};
   if (h->kernel_timing) {
     if (h->kev_used + 2 > h->kev.size()) {
       for (int q = 0; q < 256; ++q) {
@@ -195,6 +198,7 @@ int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double 
   }
   RC(launch_rhs(L.G, P, S, h->shift, h->scalar, phimode, L.slab[in], L.slab[a], L.acc, L.slab[out], h->stream));
   L.aux_phi = time0;
+  L.dtb_pre_valid = false;
   if (h->kernel_timing) {
     HIPCHK(hipEventRecord(h->kev[h->kev_used + 1], h->stream));
     h->kev_used += 2;
@@ -235,6 +239,7 @@ int do_auxiliary(oa_handle* h, int level) {
   Level& L = h->L[level];
   const bool time0 = (L.t == 0.0);
   double* y = L.slab[L.in_step ? L.last_out : L.cur];
+  L.dtb_pre_valid = false;   // derivatives are re-formed from the current state
   if (L.dirty) {
     RC(launch_project(L.G, h->phys, h->shift, h->scalar, time0, y, h->stream));
     L.dirty = false;
@@ -361,6 +366,7 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
     L.strip_z = static_cast<long>(L.G.g) * (L.G.Nr + L.G.g);
     const size_t sbytes = sizeof(double) * static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
     for (int b = 0; b < 3; ++b) ok = ok && hipMalloc(&L.strip[b], sbytes) == hipSuccess;
+    if (h->shift) ok = ok && hipMalloc(&L.dtb_pre, sizeof(double) * 2 * static_cast<size_t>(L.G.plane)) == hipSuccess;
     if (!ok) {
       oa_destroy(h);
       oa_set_error("device allocation failed");
@@ -390,6 +396,7 @@ extern "C" int oa_destroy(oa_handle* h) {
       if (L.strip[b]) (void)hipFree(L.strip[b]);
     }
     if (L.acc) (void)hipFree(L.acc);
+    if (L.dtb_pre) (void)hipFree(L.dtb_pre);
   }
   for (hipEvent_t e : h->kev) (void)hipEventDestroy(e);
   if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);
@@ -572,8 +579,16 @@ extern "C" int oa_initial_aux(oa_handle* h) {
     RC(do_auxiliary(h, l));
   }
   if (h->p.shift == OA_SHIFT_GD2 && !h->p.driverD0)
-    for (int l = 0; l < h->nlev; ++l)
-      RC(launch_dtbeta_init(h->L[l].G, h->p.drivercsi, h->scalar, h->L[l].slab[h->L[l].cur], h->stream));
+    for (int l = 0; l < h->nlev; ++l) {
+      Level& L = h->L[l];
+      // evolve.f90:174-195 sets dtbeta = drivercsi * Delta after the auxiliary
+      // pass that differentiated dtbeta: keep the planes it differentiated
+      const size_t bytes = sizeof(double) * 2 * static_cast<size_t>(L.G.plane);
+      HIPCHK(hipMemcpyAsync(L.dtb_pre, L.slab[L.cur] + static_cast<long>(shift_slot(SH_dtbeta_r)) * L.G.plane, bytes,
+                            hipMemcpyDeviceToDevice, h->stream));
+      L.dtb_pre_valid = true;
+      RC(launch_dtbeta_init(L.G, h->p.drivercsi, h->scalar, L.slab[L.cur], h->stream));
+    }
   HIPCHK(hipStreamSynchronize(h->stream));
   return OA_OK;
 }

@@ -25,17 +25,18 @@
 
 namespace oax {
 
-// ------------------------------------------------------------ RHS kernels
-// The points of a level split in two sets.  The interior, i in [1, Nr-3] and
-// j in [1, Nz-3], uses only stencils that reach at most 3 points, so each
-// 64 x 4 block stages its tile plus a 3-point halo of every field in LDS and
-// reads all stencils from there (k_rhs).  The remaining rows and columns at
-// the outer and lower-z edges use one-sided closures reaching up to 5 points
-// and read the slab directly (k_rhs_edge).  Both run the same per-point code.
-constexpr int kTileJ = 64, kTileI = 4, kHalo = 3;
-constexpr int kTileW = kTileJ + 2 * kHalo;            // tile row length (j)
-constexpr int kTileH = kTileI + 2 * kHalo;            // tile rows (i)
-constexpr int kTileSize = kTileW * kTileH;            // values per field
+// ------------------------------------------------------------ RHS kernel
+// Each 64 x 4 block stages its tile of every field in LDS and reads all
+// stencils from there.  Centred stencils reach 3 points; the one-sided
+// closures at i = Nr-1..Nr, j = Nz-1..Nz and (without equatorial symmetry)
+// j = 1-ghost..2-ghost reach up to 5 points, always towards the interior.
+// So the tile carries 5 halo rows below it in i (3 above), and 5 halo
+// columns on both sides in j, clipped to the level's (1-ghost:Nr, 1-ghost:Nz).
+constexpr int kTileJ = 64, kTileI = 4;
+constexpr int kHaloILo = 5, kHaloIHi = 3, kHaloJ = 5;
+constexpr int kTileW = kTileJ + 2 * kHaloJ;               // tile row length (j)
+constexpr int kTileH = kTileI + kHaloILo + kHaloIHi;      // tile rows (i)
+constexpr int kTileSize = kTileW * kTileH;                // values per field
 
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
 __global__ void __launch_bounds__(kTileJ * kTileI) k_rhs(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
@@ -44,54 +45,22 @@ __global__ void __launch_bounds__(kTileJ * kTileI) k_rhs(KGrid G, KPhys P, KStep
   extern __shared__ double tile[];
   constexpr int nf = nslots(SHIFT, SCALAR);
   const int i0 = 1 + static_cast<int>(blockIdx.y) * kTileI;
-  const int j0 = 1 + static_cast<int>(blockIdx.x) * kTileJ;
-  const int ib = i0 - kHalo, jb = j0 - kHalo;         // >= 1 - ghost
+  const int j0 = G.jlo + static_cast<int>(blockIdx.x) * kTileJ;
+  const int ib = i0 - kHaloILo, jb = j0 - kHaloJ;
   const int tid = static_cast<int>(threadIdx.y) * kTileJ + static_cast<int>(threadIdx.x);
-  // stage: rows ib .. ib+kTileH-1, columns jb .. jb+kTileW-1, clipped to the level
+  const int lo = 1 - G.g;
   for (int f = 0; f < nf; ++f) {
     const double* __restrict__ src = yin + f * G.plane;
     double* dst = tile + f * kTileSize;
     for (int q = tid; q < kTileSize; q += kTileJ * kTileI) {
       const int ii = ib + q / kTileW, jj = jb + q % kTileW;
-      if (ii <= G.Nr && jj <= G.Nz) dst[q] = src[IDX(G, ii, jj)];
+      if (ii >= lo && ii <= G.Nr && jj >= lo && jj <= G.Nz) dst[q] = src[IDX(G, ii, jj)];
     }
   }
   __syncthreads();
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
-  if (i > G.Nr - 3 || j > G.Nz - 3) return;
+  if (i > G.Nr || j > G.Nz) return;
   rhs_point<SHIFT, SCALAR, PHIMODE>(G, P, S, TileView{tile, ib, jb, kTileW, kTileSize}, yp, acc, yout, i, j);
-}
-
-// edge points: rows Nr-2..Nr (all j), then per interior row the columns
-// Nz-2..Nz and, without equatorial symmetry, jlo..0
-__host__ __device__ inline int edge_count(const KGrid& G) {
-  const int nj = G.Nz - G.jlo + 1;
-  return 3 * nj + (G.Nr - 3) * (3 + (1 - G.jlo));
-}
-
-OA_HD void edge_point(const KGrid& G, int q, int& i, int& j) {
-  const int nj = G.Nz - G.jlo + 1;
-  if (q < 3 * nj) {
-    i = G.Nr - 2 + q / nj;
-    j = G.jlo + q % nj;
-    return;
-  }
-  const int c = 3 + (1 - G.jlo);
-  const int qq = q - 3 * nj;
-  i = 1 + qq / c;
-  const int k = qq % c;
-  j = (k < 3) ? G.Nz - 2 + k : G.jlo + (k - 3);
-}
-
-template <bool SHIFT, bool SCALAR, bool PHIMODE>
-__global__ void __launch_bounds__(256) k_rhs_edge(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
-                                                  const double* __restrict__ yp, double* __restrict__ acc,
-                                                  double* __restrict__ yout) {
-  const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (q >= edge_count(G)) return;
-  int i, j;
-  edge_point(G, q, i, j);
-  rhs_point<SHIFT, SCALAR, PHIMODE>(G, P, S, GlobalView{yin, G}, yp, acc, yout, i, j);
 }
 
 // ------------------------------------------------------- projection pass
@@ -167,35 +136,26 @@ static int check_launch() {
 template <bool SH, bool SC, bool PM>
 static int launch_rhs_t(const KGrid& G, const KPhys& P, const KStep& S, const double* yin, const double* yp,
                         double* acc, double* yout, hipStream_t st) {
-  // interior tiles; the dynamic LDS of one block exceeds the 64 KiB default
+  // the dynamic LDS of one block exceeds the 64 KiB default
   const size_t lds = sizeof(double) * static_cast<size_t>(nslots(SH, SC)) * kTileSize;
   static bool attr_set = false;
   if (!attr_set) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rhs<SH, SC, PM>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
-      oa_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed for the RHS tile kernel");
+      oa_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed for the RHS kernel");
       return OA_ERR_HIP;
     }
     attr_set = true;
   }
   const dim3 blk(kTileJ, kTileI, 1);
-  const dim3 grd(static_cast<unsigned>((G.Nz - 3 + kTileJ - 1) / kTileJ),
-                 static_cast<unsigned>((G.Nr - 3 + kTileI - 1) / kTileI), 1);
+  const int nj = G.Nz - G.jlo + 1;
+  const dim3 grd(static_cast<unsigned>((nj + kTileJ - 1) / kTileJ), static_cast<unsigned>((G.Nr + kTileI - 1) / kTileI), 1);
   hipLaunchKernelGGL((k_rhs<SH, SC, PM>), grd, blk, lds, st, G, P, S, yin, yp, acc, yout);
-  const int rc = check_launch();
-  if (rc != OA_OK) return rc;
-  const int ne = edge_count(G);
-  hipLaunchKernelGGL((k_rhs_edge<SH, SC, PM>), dim3(static_cast<unsigned>((ne + 255) / 256)), dim3(256), 0, st,
-                     G, P, S, yin, yp, acc, yout);
   return check_launch();
 }
 
 int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar, bool phimode,
                const double* yin, const double* yp, double* acc, double* yout, void* stream) {
-  if (G.Nr < 4 || G.Nz < 4) {
-    oa_set_error("launch_rhs: the level is smaller than the 4 x 4 points the interior tiles need");
-    return OA_ERR_ARG;
-  }
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int code = (shift ? 4 : 0) + (scalar ? 2 : 0) + (phimode ? 1 : 0);
   switch (code) {

@@ -129,6 +129,22 @@ Nrtotal = 20
 Nztotal = 40
 Nl0 = 1
 """, 3),
+    "brill_gd2_odd_sizes_icn": (BASE + """
+idata = BrillWave
+brill_a0 = 2.0
+brill_sr0 = 1.0
+brill_sz0 = 1.0
+slicing = 1+log
+gauge_f = 2.0
+shift = Gammadriver2
+integrator = icn
+eqsym = .false.
+dr0 = 0.1
+dz0 = 0.1
+Nrtotal = 21
+Nztotal = 62
+Nl0 = 1
+""", 3),
     "scalar_phi2_first_icn": (BASE + """
 mattertype = scalar
 idata = scalarpulse
