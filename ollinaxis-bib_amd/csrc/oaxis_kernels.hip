@@ -26,40 +26,48 @@
 namespace oax {
 
 // ------------------------------------------------------------ RHS kernel
-// Each 64 x 4 block stages its tile of every field in LDS and reads all
-// stencils from there.  Centred stencils reach 3 points; the one-sided
-// closures at i = Nr-1..Nr, j = Nz-1..Nz and (without equatorial symmetry)
-// j = 1-ghost..2-ghost reach up to 5 points, always towards the interior.
-// So the tile carries 5 halo rows below it in i (3 above), and 5 halo
-// columns on both sides in j, clipped to the level's (1-ghost:Nr, 1-ghost:Nz).
-constexpr int kTileJ = 64, kTileI = 4;
-constexpr int kHaloILo = 5, kHaloIHi = 3, kHaloJ = 5;
-constexpr int kTileW = kTileJ + 2 * kHaloJ;               // tile row length (j)
-constexpr int kTileH = kTileI + kHaloILo + kHaloIHi;      // tile rows (i)
-constexpr int kTileSize = kTileW * kTileH;                // values per field
+// Each 16 x 16 block stages its tile of every field, with a 3-point halo, in
+// LDS and reads all stencils from there.  Centred stencils reach 3 points.
+// The one-sided closures at i = Nr-1..Nr, j = Nz-1..Nz and (without
+// equatorial symmetry) j = 1-ghost..2-ghost reach up to 5 points, always
+// towards the interior; the first and the last tile of each direction are
+// anchored to the ends of the level, so those reads stay inside the tile.
+// The last tile of a direction may overlap the one before it; its points
+// already covered there are skipped (the RK accumulator is read-modify-write).
+constexpr int kTile = 16, kHalo = 3;
+constexpr int kTileW = kTile + 2 * kHalo;                 // staged extent per direction
+constexpr int kTileSize = kTileW * kTileW;                // values per field
+
+// first point of tile b of nb along [lo, hi] (extent >= kTile)
+__host__ __device__ inline int tile_start(int b, int nb, int lo, int hi) {
+  return (b < nb - 1) ? lo + b * kTile : hi - (kTile - 1);
+}
 
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
-__global__ void __launch_bounds__(kTileJ * kTileI) k_rhs(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
-                                                         const double* __restrict__ yp, double* __restrict__ acc,
-                                                         double* __restrict__ yout) {
+__global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
+                                                          const double* __restrict__ yp, double* __restrict__ acc,
+                                                          double* __restrict__ yout) {
   extern __shared__ double tile[];
   constexpr int nf = nslots(SHIFT, SCALAR);
-  const int i0 = 1 + static_cast<int>(blockIdx.y) * kTileI;
-  const int j0 = G.jlo + static_cast<int>(blockIdx.x) * kTileJ;
-  const int ib = i0 - kHaloILo, jb = j0 - kHaloJ;
-  const int tid = static_cast<int>(threadIdx.y) * kTileJ + static_cast<int>(threadIdx.x);
+  const int nbj = static_cast<int>(gridDim.x), nbi = static_cast<int>(gridDim.y);
+  const int bj = static_cast<int>(blockIdx.x), bi = static_cast<int>(blockIdx.y);
+  const int j0 = tile_start(bj, nbj, G.jlo, G.Nz);
+  const int i0 = tile_start(bi, nbi, 1, G.Nr);
+  const int ib = i0 - kHalo, jb = j0 - kHalo;
+  const int tid = static_cast<int>(threadIdx.y) * kTile + static_cast<int>(threadIdx.x);
   const int lo = 1 - G.g;
   for (int f = 0; f < nf; ++f) {
     const double* __restrict__ src = yin + f * G.plane;
     double* dst = tile + f * kTileSize;
-    for (int q = tid; q < kTileSize; q += kTileJ * kTileI) {
+    for (int q = tid; q < kTileSize; q += kTile * kTile) {
       const int ii = ib + q / kTileW, jj = jb + q % kTileW;
       if (ii >= lo && ii <= G.Nr && jj >= lo && jj <= G.Nz) dst[q] = src[IDX(G, ii, jj)];
     }
   }
   __syncthreads();
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
-  if (i > G.Nr || j > G.Nz) return;
+  if (bi == nbi - 1 && i < 1 + (nbi - 1) * kTile) return;        // done by the tile before
+  if (bj == nbj - 1 && j < G.jlo + (nbj - 1) * kTile) return;
   rhs_point<SHIFT, SCALAR, PHIMODE>(G, P, S, TileView{tile, ib, jb, kTileW, kTileSize}, yp, acc, yout, i, j);
 }
 
@@ -147,15 +155,19 @@ static int launch_rhs_t(const KGrid& G, const KPhys& P, const KStep& S, const do
     }
     attr_set = true;
   }
-  const dim3 blk(kTileJ, kTileI, 1);
+  const dim3 blk(kTile, kTile, 1);
   const int nj = G.Nz - G.jlo + 1;
-  const dim3 grd(static_cast<unsigned>((nj + kTileJ - 1) / kTileJ), static_cast<unsigned>((G.Nr + kTileI - 1) / kTileI), 1);
+  const dim3 grd(static_cast<unsigned>((nj + kTile - 1) / kTile), static_cast<unsigned>((G.Nr + kTile - 1) / kTile), 1);
   hipLaunchKernelGGL((k_rhs<SH, SC, PM>), grd, blk, lds, st, G, P, S, yin, yp, acc, yout);
   return check_launch();
 }
 
 int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar, bool phimode,
                const double* yin, const double* yp, double* acc, double* yout, void* stream) {
+  if (G.Nr < kTile || G.Nz - G.jlo + 1 < kTile) {
+    oa_set_error("launch_rhs: a level needs at least 16 points in each direction");
+    return OA_ERR_ARG;
+  }
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int code = (shift ? 4 : 0) + (scalar ? 2 : 0) + (phimode ? 1 : 0);
   switch (code) {

@@ -112,9 +112,12 @@ def supported(text):
 
 
 def test_supported_configurations():
-    assert supported("order = four\n")[0] == 0
+    size = "Nrtotal = 32\nNztotal = 32\n"
+    assert supported("order = four\n" + size)[0] == 0
     assert supported(SCHWARZ)[0] == 0
-    assert supported("order = four\nshift = Gammadriver2\nmattertype = scalar\n")[0] == 0
+    assert supported("order = four\nshift = Gammadriver2\nmattertype = scalar\n" + size)[0] == 0
+    # the reference's default 10 x 10 grid is below one 16 x 16 tile
+    assert supported("order = four\n")[0] != 0
 
 
 @pytest.mark.parametrize("text, word", [
