@@ -78,6 +78,7 @@ struct KTime {
 };
 
 // launchers (oaxis_kernels.hip)
+// One RK substep of a level (all its points).
 int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar,
                bool phimode, const double* yin, const double* yp, double* acc, double* yout,
                void* stream);

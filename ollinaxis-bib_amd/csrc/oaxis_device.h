@@ -25,11 +25,13 @@ using std::exp; using std::log; using std::sqrt; using std::cbrt; using std::fab
 // The closures follow derivatives.f90:67-88 (diff1r), 179-212 (diff1z),
 // 308-330 (diff2r), 425-459 (diff2z) and 606-691 (diff2rz) for the point
 // classes that are computed (i >= 1; j >= 1 with equatorial symmetry).
-// F is any callable returning the value at (ii, jj).
+// F is any callable returning the value at (ii, jj).  With IN = true the
+// point is known to lie in i in [1, Nr-3], j in [1, Nz-3], where every
+// stencil takes its centred (interior) form, and the closures compile out.
 
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_d1r(const F& f, const KGrid& G, int i, int j) {
-  if (i <= G.Nr - 2)
+  if (IN || i <= G.Nr - 2)
     return G.idr * (8.0 * (f(i + 1, j) - f(i - 1, j)) - (f(i + 2, j) - f(i - 2, j))) * c12;
   if (i == G.Nr - 1)
     return G.idr * (3.0 * f(i + 1, j) + 10.0 * f(i, j) - 18.0 * f(i - 1, j) + 6.0 * f(i - 2, j)
@@ -38,9 +40,9 @@ OA_HD double s_d1r(const F& f, const KGrid& G, int i, int j) {
                   + 3.0 * f(i - 4, j)) * c12;
 }
 
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_d1z(const F& f, const KGrid& G, int i, int j) {
-  if (j >= 0 && j <= G.Nz - 2)
+  if (IN || (j >= 0 && j <= G.Nz - 2))
     return G.idz * (8.0 * (f(i, j + 1) - f(i, j - 1)) - (f(i, j + 2) - f(i, j - 2))) * c12;
   if (j == G.Nz - 1)
     return G.idz * (3.0 * f(i, j + 1) + 10.0 * f(i, j) - 18.0 * f(i, j - 1) + 6.0 * f(i, j - 2)
@@ -55,10 +57,10 @@ OA_HD double s_d1z(const F& f, const KGrid& G, int i, int j) {
                    + 3.0 * f(i, j + 4)) * c12;
 }
 
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_d2r(const F& f, const KGrid& G, int i, int j) {
   const double idr2 = G.idr * G.idr;
-  if (i <= G.Nr - 2)
+  if (IN || i <= G.Nr - 2)
     return -idr2 * (30.0 * f(i, j) - 16.0 * (f(i + 1, j) + f(i - 1, j)) + f(i + 2, j) + f(i - 2, j)) * c12;
   if (i == G.Nr - 1)
     return idr2 * (10.0 * f(i + 1, j) - 15.0 * f(i, j) - 4.0 * f(i - 1, j) + 14.0 * f(i - 2, j)
@@ -67,10 +69,10 @@ OA_HD double s_d2r(const F& f, const KGrid& G, int i, int j) {
                  + 61.0 * f(i - 4, j) - 10.0 * f(i - 5, j)) * c12;
 }
 
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_d2z(const F& f, const KGrid& G, int i, int j) {
   const double idz2 = G.idz * G.idz;
-  if (j >= 0 && j <= G.Nz - 2)
+  if (IN || (j >= 0 && j <= G.Nz - 2))
     return -idz2 * (30.0 * f(i, j) - 16.0 * (f(i, j + 1) + f(i, j - 1)) + f(i, j + 2) + f(i, j - 2)) * c12;
   if (j == G.Nz - 1)
     return idz2 * (10.0 * f(i, j + 1) - 15.0 * f(i, j) - 4.0 * f(i, j - 1) + 14.0 * f(i, j - 2)
@@ -87,9 +89,12 @@ OA_HD double s_d2z(const F& f, const KGrid& G, int i, int j) {
 
 // mixed derivative: 4th order inside, the reference's 2nd-order edge rows /
 // columns and corners, resolved per point
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_d2rz(const F& f, const KGrid& G, int i, int j) {
   const double idrz = G.idr * G.idz;
+  if constexpr (IN)
+    return idrz * ((f(i + 1, j + 1) + f(i - 1, j - 1) - f(i + 1, j - 1) - f(i - 1, j + 1)) * third
+                   - (f(i + 2, j + 2) + f(i - 2, j - 2) - f(i + 2, j - 2) - f(i - 2, j + 2)) * (1.0 / 48.0));
   const bool zedge = (j == -2 || j == G.Nz);
   const bool znear = (j == -1 || j == G.Nz - 1);
   if (i == G.Nr) {
@@ -118,9 +123,16 @@ OA_HD double s_d2rz(const F& f, const KGrid& G, int i, int j) {
 }
 
 // upwinded advective derivatives, derivadvect.f90:105-175 / 304-374
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_adr(const F& f, const KGrid& G, int i, int j, double beta) {
   const double q = 0.25 * G.idr * third;
+  if constexpr (IN) {
+    // both upwind sides in one expression: s = +1 reads i+1..i-3, s = -1 reads
+    // i-1..i+3 with the overall sign flipped (s*q is exact)
+    const int s = (beta <= 0.0) ? 1 : -1;
+    return (s * q) * (3.0 * f(i + s, j) + 10.0 * f(i, j) - 18.0 * f(i - s, j) + 6.0 * f(i - 2 * s, j)
+                      - f(i - 3 * s, j));
+  }
   if (i <= G.Nr - 3) {
     if (beta <= 0.0)
       return q * (3.0 * f(i + 1, j) + 10.0 * f(i, j) - 18.0 * f(i - 1, j) + 6.0 * f(i - 2, j) - f(i - 3, j));
@@ -136,9 +148,14 @@ OA_HD double s_adr(const F& f, const KGrid& G, int i, int j, double beta) {
   return q * (25.0 * f(i, j) - 48.0 * f(i - 1, j) + 36.0 * f(i - 2, j) - 16.0 * f(i - 3, j) + 3.0 * f(i - 4, j));
 }
 
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_adz(const F& f, const KGrid& G, int i, int j, double beta) {
   const double q = 0.25 * G.idz * third;
+  if constexpr (IN) {
+    const int s = (beta <= 0.0) ? 1 : -1;
+    return (s * q) * (3.0 * f(i, j + s) + 10.0 * f(i, j) - 18.0 * f(i, j - s) + 6.0 * f(i, j - 2 * s)
+                      - f(i, j - 3 * s));
+  }
   if (j >= 1 && j <= G.Nz - 3) {
     if (beta <= 0.0)
       return q * (3.0 * f(i, j + 1) + 10.0 * f(i, j) - 18.0 * f(i, j - 1) + 6.0 * f(i, j - 2) - f(i, j - 3));
@@ -165,14 +182,14 @@ OA_HD double s_adz(const F& f, const KGrid& G, int i, int j, double beta) {
 
 // Kreiss-Oliger D^6 terms of dissipation.f90:185-199: returns the amount to
 // subtract from the source (already scaled by diss/dt)
-template <class F>
+template <bool IN = false, class F>
 OA_HD double s_ko(const F& f, const KGrid& G, int i, int j, double dscale) {
   double s = 0.0;
-  if (i <= G.Nr - 3)
+  if (IN || i <= G.Nr - 3)
     s = dscale * (20.0 * f(i, j) - 15.0 * (f(i + 1, j) + f(i - 1, j)) + 6.0 * (f(i + 2, j) + f(i - 2, j))
                   - (f(i + 3, j) + f(i - 3, j))) * 0.05;
   double t = 0.0;
-  if (j >= 1 && j <= G.Nz - 3)
+  if (IN || (j >= 1 && j <= G.Nz - 3))
     t = dscale * (20.0 * f(i, j) - 15.0 * (f(i, j + 1) + f(i, j - 1)) + 6.0 * (f(i, j + 2) + f(i, j - 2))
                   - (f(i, j + 3) + f(i, j - 3))) * 0.05;
   return s + t;
@@ -188,10 +205,15 @@ struct Plane {
 // fields through V.fld(slot)(i, j).  GlobalView reads the level slab in HBM;
 // TileView reads a block's tile (with a 3-point halo) staged in LDS, laid out
 // row-major as (i - ib) * tw + (j - jb), one tile of ts values per field.
+// prefld(k) reads plane k (0: dtbeta_r, 1: dtbeta_z) of the dtbeta state that
+// the first substep after the initial auxiliary pass differentiates
+// (KStep::dtb_pre).
 struct GlobalView {
   const double* __restrict__ y;
   KGrid G;
+  const double* pre;
   OA_HD Plane fld(int s) const { return Plane{y + s * G.plane, G}; }
+  OA_HD Plane prefld(int k) const { return Plane{pre + k * G.plane, G}; }
 };
 
 struct TilePlane {
@@ -203,19 +225,22 @@ struct TilePlane {
 struct TileView {
   const double* t;
   int ib, jb, tw, ts;
+  int npre;             // tile slot of prefld(0): staged after the evolved fields
   OA_HD TilePlane fld(int s) const { return TilePlane{t + s * ts, ib, jb, tw}; }
+  OA_HD TilePlane prefld(int k) const { return TilePlane{t + (npre + k) * ts, ib, jb, tw}; }
 };
 
 struct D1 { double r, z; };
 struct D2 { double r, z, rr, zz, rz; };
 
-template <class F>
+template <bool IN = false, class F>
 OA_HD D1 grad(const F& f, const KGrid& G, int i, int j) {
-  return {s_d1r(f, G, i, j), s_d1z(f, G, i, j)};
+  return {s_d1r<IN>(f, G, i, j), s_d1z<IN>(f, G, i, j)};
 }
-template <class F>
+template <bool IN = false, class F>
 OA_HD D2 hess(const F& f, const KGrid& G, int i, int j) {
-  return {s_d1r(f, G, i, j), s_d1z(f, G, i, j), s_d2r(f, G, i, j), s_d2z(f, G, i, j), s_d2rz(f, G, i, j)};
+  return {s_d1r<IN>(f, G, i, j), s_d1z<IN>(f, G, i, j), s_d2r<IN>(f, G, i, j), s_d2z<IN>(f, G, i, j),
+          s_d2rz<IN>(f, G, i, j)};
 }
 
 // ------------------------------------------------------------- projection
@@ -273,7 +298,7 @@ OA_HD void slot_parity(bool shift, bool scalar, int s, double& symr, double& sym
 // never formed (they keep their initial zero).  This holds for the first step
 // of a level and for the first internal iteration of its second step.
 // S.proj_t0: the auxiliary call that ends this iteration runs at clock 0.
-template <bool SHIFT, bool SCALAR, bool PHIMODE, class View>
+template <bool SHIFT, bool SCALAR, bool PHIMODE, bool IN, class View>
 OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View& V,
                      const double* __restrict__ yp, double* __restrict__ acc, double* __restrict__ yout,
                      int i, int j) {
@@ -309,8 +334,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   // advective derivative: upwinded (shiftadvect) or the centred one
   auto adv = [&](const auto& f, const D1& d, double& DAr, double& DAz) {
     if (P.shiftadvect) {
-      DAr = s_adr(f, G, i, j, br);
-      DAz = s_adz(f, G, i, j, bz);
+      DAr = s_adr<IN>(f, G, i, j, br);
+      DAz = s_adz<IN>(f, G, i, j, bz);
     } else {
       DAr = d.r;
       DAz = d.z;
@@ -318,7 +343,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   };
 
   // ---- lapse (auxiliary_geometry.f90:55-72)
-  const D2 da = hess(Fa, G, i, j);
+  const D2 da = hess<IN>(Fa, G, i, j);
   double DAr_alpha = 0.0, DAz_alpha = 0.0;
   if constexpr (SHIFT) adv(Fa, D1{da.r, da.z}, DAr_alpha, DAz_alpha);
 
@@ -327,10 +352,10 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   D2 dp;
   double DAr_phi = 0.0, DAz_phi = 0.0, DAr_chi = 0.0, DAz_chi = 0.0;
   if constexpr (PHIMODE) {
-    dp = hess(Fphi, G, i, j);
+    dp = hess<IN>(Fphi, G, i, j);
     if constexpr (SHIFT) adv(Fphi, D1{dp.r, dp.z}, DAr_phi, DAz_phi);
   } else {
-    const D2 dc = hess(Fchi, G, i, j);
+    const D2 dc = hess<IN>(Fchi, G, i, j);
     const double ic = 1.0 / chi;
     const double qr = dc.r * ic, qz = dc.z * ic;
     dp.r = -0.5 * qr;
@@ -346,13 +371,13 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   }
 
   // ---- conformal metric and its inverse (:325-507); hdet = 1
-  const D2 dA = hess(FA, G, i, j), dB = hess(FB, G, i, j), dH = hess(FH, G, i, j), dC = hess(FC, G, i, j);
+  const D2 dA = hess<IN>(FA, G, i, j), dB = hess<IN>(FB, G, i, j), dH = hess<IN>(FH, G, i, j), dC = hess<IN>(FC, G, i, j);
   const double gA = B * H, gB = A * H, gH = 1.0 / H, gC = -C * H;
   auto fgA = [&](int ii, int jj) { return FB(ii, jj) * FH(ii, jj); };
   auto fgB = [&](int ii, int jj) { return FA(ii, jj) * FH(ii, jj); };
   auto fgH = [&](int ii, int jj) { return 1.0 / FH(ii, jj); };
   auto fgC = [&](int ii, int jj) { return -FC(ii, jj) * FH(ii, jj); };
-  const D1 dgA = grad(fgA, G, i, j), dgB = grad(fgB, G, i, j), dgC = grad(fgC, G, i, j);
+  const D1 dgA = grad<IN>(fgA, G, i, j), dgB = grad<IN>(fgB, G, i, j), dgC = grad<IN>(fgC, G, i, j);
   (void)fgH;
 
   // ---- KT_H, regularisation (:747-821)
@@ -362,7 +387,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   const double g_lambda = C * C - B * lambda;
 
   // ---- extrinsic curvature derivatives (:830-906)
-  const D1 dK = grad(FK, G, i, j), dKA = grad(FKA, G, i, j), dKB = grad(FKB, G, i, j), dKC = grad(FKC, G, i, j);
+  const D1 dK = grad<IN>(FK, G, i, j), dKA = grad<IN>(FKA, G, i, j), dKB = grad<IN>(FKB, G, i, j), dKC = grad<IN>(FKC, G, i, j);
 
   // ---- contravariant KT, KT2, K2 (:970-1050)
   const double gA2 = gA * gA, gB2 = gB * gB, gH2 = gH * gH, gC2 = gC * gC;
@@ -380,17 +405,17 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   // ---- Delta (:1085-1175)
   const double DelDef_r = -dgA.r - r * dgC.z - r * g_lambda;
   const double DelDef_z = -dgC.r * r - dgB.z - 2.0 * gC;
-  const D1 dDr = grad(FDr, G, i, j), dDz = grad(FDz, G, i, j);
+  const D1 dDr = grad<IN>(FDr, G, i, j), dDz = grad<IN>(FDz, G, i, j);
 
   // ---- shift derivatives (:587-737)
   D2 dbr_{}, dbz_{};
   double DD_beta_rr = 0.0, DIV_beta = 0.0, Dr_DIV_beta = 0.0, Dz_DIV_beta = 0.0;
   if constexpr (SHIFT) {
     const auto Fbr = fld(shift_slot(SH_beta_r)), Fbz = fld(shift_slot(SH_beta_z));
-    dbr_ = hess(Fbr, G, i, j);
-    dbz_ = hess(Fbz, G, i, j);
+    dbr_ = hess<IN>(Fbr, G, i, j);
+    dbz_ = hess<IN>(Fbz, G, i, j);
     auto fbr_r = [&](int ii, int jj) { return Fbr(ii, jj) / ((static_cast<double>(ii) - 0.5) * G.dr); };
-    DD_beta_rr = s_d1r(fbr_r, G, i, j);
+    DD_beta_rr = s_d1r<IN>(fbr_r, G, i, j);
     DIV_beta = dbr_.r + dbz_.z + br * ir;
     Dr_DIV_beta = dbr_.rr + dbz_.rz + DD_beta_rr;
     Dz_DIV_beta = dbr_.rz + dbz_.zz + dbr_.z * ir;
@@ -582,24 +607,24 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     trS = (gA * mS_A + gB * mS_B + gH * mS_H + 2.0 * r2 * gC * mS_C) * ipsi4;
 
     // sources_scalar (scalar.f90:94-251)
-    const D1 dpi = grad(Fpi, G, i, j);
+    const D1 dpi = grad<IN>(Fpi, G, i, j);
     s_sphi = alpha * pi;
     if constexpr (SHIFT) s_sphi += br * xr + bz * xz;
     s_sxr = alpha * dpi.r + pi * Dra;
     s_sxz = alpha * dpi.z + pi * Dza;
     if constexpr (SHIFT) {
-      const D1 dxr = grad(Fxr, G, i, j), dxz = grad(Fxz, G, i, j);
-      s_sxr += br * s_adr(Fxr, G, i, j, br) + xr * dbr_.r + bz * s_adz(Fxr, G, i, j, bz) + xz * dbz_.r;
-      s_sxz += br * s_adr(Fxz, G, i, j, br) + xr * dbr_.z + bz * s_adz(Fxz, G, i, j, bz) + xz * dbz_.z;
+      const D1 dxr = grad<IN>(Fxr, G, i, j), dxz = grad<IN>(Fxz, G, i, j);
+      s_sxr += br * s_adr<IN>(Fxr, G, i, j, br) + xr * dbr_.r + bz * s_adz<IN>(Fxr, G, i, j, bz) + xz * dbz_.r;
+      s_sxz += br * s_adr<IN>(Fxz, G, i, j, br) + xr * dbr_.z + bz * s_adz<IN>(Fxz, G, i, j, bz) + xz * dbz_.z;
       (void)dxr; (void)dxz;
     }
     double dr_, dz_, lap;
     if (P.scalarmethod == OA_SCALARMETHOD_FIRST) {
-      const D1 dxr = grad(Fxr, G, i, j), dxz = grad(Fxz, G, i, j);
+      const D1 dxr = grad<IN>(Fxr, G, i, j), dxz = grad<IN>(Fxz, G, i, j);
       dr_ = xr; dz_ = xz;
       lap = alpha * (gA * dxr.r + gB * dxz.z + r * gC * (dxz.r + dxr.z));
     } else {
-      const D2 ds = hess(Fsp, G, i, j);
+      const D2 ds = hess<IN>(Fsp, G, i, j);
       dr_ = ds.r; dz_ = ds.z;
       lap = alpha * (gA * ds.rr + gB * ds.zz + 2.0 * r * gC * ds.rz);
     }
@@ -611,16 +636,16 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     spi = spi * ipsi4;
     spi += alpha * pi * trK;
     if (P.scalarpotential != OA_POT_NONE) spi -= alpha * VP;
-    if constexpr (SHIFT) spi += br * s_adr(Fpi, G, i, j, br) + bz * s_adz(Fpi, G, i, j, bz);
+    if constexpr (SHIFT) spi += br * s_adr<IN>(Fpi, G, i, j, br) + bz * s_adz<IN>(Fpi, G, i, j, bz);
     if (P.scalardiss != 0.0) {
-      s_sphi -= s_ko(Fsp, G, i, j, dscale_s);
-      spi -= s_ko(Fpi, G, i, j, dscale_s);
-      s_sxr -= s_ko(Fxr, G, i, j, dscale_s);
-      s_sxz -= s_ko(Fxz, G, i, j, dscale_s);
+      s_sphi -= s_ko<IN>(Fsp, G, i, j, dscale_s);
+      spi -= s_ko<IN>(Fpi, G, i, j, dscale_s);
+      s_sxr -= s_ko<IN>(Fxr, G, i, j, dscale_s);
+      s_sxz -= s_ko<IN>(Fxz, G, i, j, dscale_s);
     }
     // radiative boundary on pi at every level (scalar.f90:244-251)
     const bool bnd = (i == G.Nr) || (j == G.Nz) || (!G.eqsym && j == 1 - G.g);
-    if (bnd) {
+    if (!IN && bnd) {
       const double rr = sqrt(r * r + z * z);
       spi = -(r * dpi.r + z * dpi.z + pi) / rr;
     }
@@ -661,7 +686,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
       strK += br * DAr + bz * DAz;
     }
     if constexpr (SCALAR) strK += 4.0 * smallpi * alpha * (rho + trS);
-    if (P.geodiss != 0.0) strK -= s_ko(FK, G, i, j, dscale_g);
+    if (P.geodiss != 0.0) strK -= s_ko<IN>(FK, G, i, j, dscale_g);
   }
 
   // traceless extrinsic curvature
@@ -688,9 +713,9 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     sKTC -= 8.0 * smallpi * alpha * (mS_C * ipsi4 - third * C * trS);
   }
   if (P.geodiss != 0.0) {
-    sKTA -= s_ko(FKA, G, i, j, dscale_g);
-    sKTB -= s_ko(FKB, G, i, j, dscale_g);
-    sKTC -= s_ko(FKC, G, i, j, dscale_g);
+    sKTA -= s_ko<IN>(FKA, G, i, j, dscale_g);
+    sKTB -= s_ko<IN>(FKB, G, i, j, dscale_g);
+    sKTC -= s_ko<IN>(FKC, G, i, j, dscale_g);
   }
 
   // Delta
@@ -716,8 +741,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     sDz -= 8.0 * smallpi * eta * alpha * J_z * psi4;
   }
   if (P.geodiss != 0.0) {
-    sDr -= s_ko(FDr, G, i, j, dscale_g);
-    sDz -= s_ko(FDz, G, i, j, dscale_g);
+    sDr -= s_ko<IN>(FDr, G, i, j, dscale_g);
+    sDz -= s_ko<IN>(FDz, G, i, j, dscale_g);
   }
 
   // ---- sources_lapse (first-order Bona-Masso)
@@ -734,10 +759,11 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
 
   // ---- sources_shift (Gammadriver2)
   // the planes dtbeta is differentiated from (see KStep::dtb_pre)
-  const double* const dpre = S.dtb_pre;
+  const bool dpre = (S.dtb_pre != nullptr);
   const auto Fdbr = fld(SHIFT ? shift_slot(SH_dtbeta_r) : 0), Fdbz = fld(SHIFT ? shift_slot(SH_dtbeta_z) : 0);
-  auto Ddbr = [&](int ii, int jj) { return dpre ? dpre[IDX(G, ii, jj)] : Fdbr(ii, jj); };
-  auto Ddbz = [&](int ii, int jj) { return dpre ? dpre[pl + IDX(G, ii, jj)] : Fdbz(ii, jj); };
+  const auto Pdbr = V.prefld(0), Pdbz = V.prefld(1);
+  auto Ddbr = [&](int ii, int jj) { return dpre ? Pdbr(ii, jj) : Fdbr(ii, jj); };
+  auto Ddbz = [&](int ii, int jj) { return dpre ? Pdbz(ii, jj) : Fdbz(ii, jj); };
   double sbr = 0.0, sbz = 0.0, sdbr = 0.0, sdbz = 0.0;
   if constexpr (SHIFT) {
     sbr = dbr;
@@ -755,10 +781,10 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     sdbz = P.drivercsi * sDz;
     if (P.driveradv) {
       double tr_, tz_, dr_, dz_;
-      adv(Ddbr, grad(Ddbr, G, i, j), tr_, tz_);
+      adv(Ddbr, grad<IN>(Ddbr, G, i, j), tr_, tz_);
       adv(FDr, dDr, dr_, dz_);
       sdbr += br * (tr_ - dr_) + bz * (tz_ - dz_);
-      adv(Ddbz, grad(Ddbz, G, i, j), tr_, tz_);
+      adv(Ddbz, grad<IN>(Ddbz, G, i, j), tr_, tz_);
       adv(FDz, dDz, dr_, dz_);
       sdbz += br * (tr_ - dr_) + bz * (tz_ - dz_);
     }
@@ -767,7 +793,7 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   }
 
   // ---- radiative boundaries on level 0 (radiative_geometry.f90:58-237)
-  if (P.radiative) {
+  if (!IN && P.radiative) {
     const bool onr = (i == G.Nr), onzR = (j == G.Nz), onzL = (!G.eqsym && j == 1 - G.g);
     if (onr || onzR || onzL) {
       const double irr = 1.0 / sqrt(r * r + z * z);
@@ -783,8 +809,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
       if constexpr (SHIFT) {
         if (P.drivercsi != 0.0) {
           const double vs = sqrt(4.0 * third * P.drivercsi);
-          const D1 ddr = grad(Ddbr, G, i, j);
-          const D1 ddz = grad(Ddbz, G, i, j);
+          const D1 ddr = grad<IN>(Ddbr, G, i, j);
+          const D1 ddz = grad<IN>(Ddbz, G, i, j);
           sdbr = -vs * (r * ddr.r + z * ddr.z + dbr) * irr;
           sdbz = -vs * (r * ddz.r + z * ddz.z + dbz) * irr;
         }

@@ -56,8 +56,10 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KSte
   const int ib = i0 - kHalo, jb = j0 - kHalo;
   const int tid = static_cast<int>(threadIdx.y) * kTile + static_cast<int>(threadIdx.x);
   const int lo = 1 - G.g;
-  for (int f = 0; f < nf; ++f) {
-    const double* __restrict__ src = yin + f * G.plane;
+  // the evolved fields, then (first substep only) the dtbeta planes it differentiates
+  const int nst = nf + (S.dtb_pre ? 2 : 0);
+  for (int f = 0; f < nst; ++f) {
+    const double* __restrict__ src = (f < nf) ? yin + f * G.plane : S.dtb_pre + (f - nf) * G.plane;
     double* dst = tile + f * kTileSize;
     for (int q = tid; q < kTileSize; q += kTile * kTile) {
       const int ii = ib + q / kTileW, jj = jb + q % kTileW;
@@ -68,7 +70,11 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KSte
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
   if (bi == nbi - 1 && i < 1 + (nbi - 1) * kTile) return;        // done by the tile before
   if (bj == nbj - 1 && j < G.jlo + (nbj - 1) * kTile) return;
-  rhs_point<SHIFT, SCALAR, PHIMODE>(G, P, S, TileView{tile, ib, jb, kTileW, kTileSize}, yp, acc, yout, i, j);
+  // the general stencils (with their closure branches) everywhere: a
+  // closure-free interior instantiation schedules more loads early and
+  // spills at two waves per SIMD
+  rhs_point<SHIFT, SCALAR, PHIMODE, false>(G, P, S, TileView{tile, ib, jb, kTileW, kTileSize, nf}, yp, acc, yout,
+                                           i, j);
 }
 
 // ------------------------------------------------------- projection pass
@@ -144,12 +150,15 @@ static int check_launch() {
 template <bool SH, bool SC, bool PM>
 static int launch_rhs_t(const KGrid& G, const KPhys& P, const KStep& S, const double* yin, const double* yp,
                         double* acc, double* yout, hipStream_t st) {
-  // the dynamic LDS of one block exceeds the 64 KiB default
-  const size_t lds = sizeof(double) * static_cast<size_t>(nslots(SH, SC)) * kTileSize;
+  // the dynamic LDS of one block exceeds the 64 KiB default; two more
+  // planes on a level's first substep (KStep::dtb_pre)
+  const size_t per = sizeof(double) * kTileSize;
+  const size_t lds = per * static_cast<size_t>(nslots(SH, SC) + (S.dtb_pre ? 2 : 0));
   static bool attr_set = false;
   if (!attr_set) {
+    const size_t most = per * static_cast<size_t>(nslots(SH, SC) + 2);
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rhs<SH, SC, PM>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(most)) != hipSuccess) {
       oa_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed for the RHS kernel");
       return OA_ERR_HIP;
     }
