@@ -23,6 +23,15 @@ enum ShiftSlot { SH_beta_r = 0, SH_beta_z, SH_dtbeta_r, SH_dtbeta_z, NSHIFT };
 static_assert(SH_dtbeta_z == SH_dtbeta_r + 1, "dtbeta_r and dtbeta_z planes are adjacent");
 enum ScalarSlot { SC_phi = 0, SC_xi_r, SC_xi_z, SC_pi, NSCALAR };
 
+// Planes of the per-level buffer of geometric auxiliaries that the first RHS
+// kernel of a substep writes and the second reads: D_i D_j alpha (A, B, C
+// components and the Laplacian), the Ricci tensor (A, B, C components and the
+// scalar), and the divergences DIV_KT, DIVA_KT.
+enum AuxSlot {
+  AUX_d2A = 0, AUX_d2B, AUX_d2C, AUX_Lapla, AUX_RA, AUX_RB, AUX_RC, AUX_RSCAL,
+  AUX_DIV_r, AUX_DIV_z, AUX_DIVA_r, AUX_DIVA_z, NAUX
+};
+
 constexpr int nslots(bool shift, bool scalar) {
   return NBASE + (shift ? NSHIFT : 0) + (scalar ? NSCALAR : 0);
 }
@@ -78,10 +87,11 @@ struct KTime {
 };
 
 // launchers (oaxis_kernels.hip)
-// One RK substep of a level (all its points).
+// One RK substep of a level (all its points); gaux holds NAUX planes of
+// per-point geometric auxiliaries passed between the two RHS kernels.
 int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar,
                bool phimode, const double* yin, const double* yp, double* acc, double* yout,
-               void* stream);
+               double* gaux, void* stream);
 int launch_project(const KGrid& G, const KPhys& P, bool shift, bool scalar, bool time0,
                    double* y, void* stream);
 int launch_deldef(const KGrid& G, double* y, void* stream);

@@ -32,6 +32,7 @@ struct Level {
   bool dirty = true;                                // state needs the full projection
   bool phi_valid = true;                            // phi plane holds the reference's phi
   bool aux_phi = true;                              // last auxiliary ran at clock 0 (phi mode)
+  double* gaux = nullptr;                           // geometric auxiliaries between the two RHS kernels
   double* dtb_pre = nullptr;                        // dtbeta_r, dtbeta_z before the driver initialisation
   bool dtb_pre_valid = false;                       // the next substep differentiates dtb_pre
   double t = 0.0, t1 = 0.0, t2 = 0.0;
@@ -196,7 +197,8 @@ int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double 
     }
     HIPCHK(hipEventRecord(h->kev[h->kev_used], h->stream));
   }
-  RC(launch_rhs(L.G, P, S, h->shift, h->scalar, phimode, L.slab[in], L.slab[a], L.acc, L.slab[out], h->stream));
+  RC(launch_rhs(L.G, P, S, h->shift, h->scalar, phimode, L.slab[in], L.slab[a], L.acc, L.slab[out], L.gaux,
+                h->stream));
   L.aux_phi = time0;
   L.dtb_pre_valid = false;
   if (h->kernel_timing) {
@@ -367,6 +369,7 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
     const size_t sbytes = sizeof(double) * static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
     for (int b = 0; b < 3; ++b) ok = ok && hipMalloc(&L.strip[b], sbytes) == hipSuccess;
     if (h->shift) ok = ok && hipMalloc(&L.dtb_pre, sizeof(double) * 2 * static_cast<size_t>(L.G.plane)) == hipSuccess;
+    ok = ok && hipMalloc(&L.gaux, sizeof(double) * NAUX * static_cast<size_t>(L.G.plane)) == hipSuccess;
     if (!ok) {
       oa_destroy(h);
       oa_set_error("device allocation failed");
@@ -397,6 +400,7 @@ extern "C" int oa_destroy(oa_handle* h) {
     }
     if (L.acc) (void)hipFree(L.acc);
     if (L.dtb_pre) (void)hipFree(L.dtb_pre);
+    if (L.gaux) (void)hipFree(L.gaux);
   }
   for (hipEvent_t e : h->kev) (void)hipEventDestroy(e);
   if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);

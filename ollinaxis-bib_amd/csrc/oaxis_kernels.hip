@@ -43,10 +43,39 @@ __host__ __device__ inline int tile_start(int b, int nb, int lo, int hi) {
   return (b < nb - 1) ? lo + b * kTile : hi - (kTile - 1);
 }
 
+// A tile wholly inside i in [1, Nr-3], j in [1, Nz-3]: every stencil there
+// takes its centred form, so the closure branches compile out (IN).
+__device__ __forceinline__ bool interior_tile(const KGrid& G, int i0, int j0) {
+  return i0 >= 1 && i0 + kTile - 1 <= G.Nr - 3 && j0 >= 1 && j0 + kTile - 1 <= G.Nz - 3;
+}
+
+// Stage one RHS input tile: nst planes, plane f from src(f), into LDS.
+template <class Src>
+__device__ __forceinline__ void stage_tile(const KGrid& G, double* tile, int nst, Src src, int ib, int jb) {
+  const int tid = static_cast<int>(threadIdx.y) * kTile + static_cast<int>(threadIdx.x);
+  const int lo = 1 - G.g;
+  for (int f = 0; f < nst; ++f) {
+    const double* __restrict__ p = src(f);
+    double* dst = tile + f * kTileSize;
+    for (int q = tid; q < kTileSize; q += kTile * kTile) {
+      const int ii = ib + q / kTileW, jj = jb + q % kTileW;
+      if (ii >= lo && ii <= G.Nr && jj >= lo && jj <= G.Nz) dst[q] = p[IDX(G, ii, jj)];
+    }
+  }
+  __syncthreads();
+}
+
+// One RK substep of a level: each 16 x 16 block stages its tile and runs two
+// phases per point.  Phase 1 forms the geometric auxiliaries (D_i D_j alpha,
+// the Ricci tensor, div KT: the longest expressions, which need only the
+// vacuum fields) and passes them through the per-level buffer gaux; phase 2
+// forms every source, the dissipation, the radiative boundaries and the RK
+// update.  Kept apart, the phases need at most the registers of the larger
+// one, so the kernel runs at two waves per SIMD without spilling.
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
 __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
                                                           const double* __restrict__ yp, double* __restrict__ acc,
-                                                          double* __restrict__ yout) {
+                                                          double* __restrict__ yout, double* __restrict__ gaux) {
   extern __shared__ double tile[];
   constexpr int nf = nslots(SHIFT, SCALAR);
   const int nbj = static_cast<int>(gridDim.x), nbi = static_cast<int>(gridDim.y);
@@ -54,27 +83,24 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KSte
   const int j0 = tile_start(bj, nbj, G.jlo, G.Nz);
   const int i0 = tile_start(bi, nbi, 1, G.Nr);
   const int ib = i0 - kHalo, jb = j0 - kHalo;
-  const int tid = static_cast<int>(threadIdx.y) * kTile + static_cast<int>(threadIdx.x);
-  const int lo = 1 - G.g;
   // the evolved fields, then (first substep only) the dtbeta planes it differentiates
   const int nst = nf + (S.dtb_pre ? 2 : 0);
-  for (int f = 0; f < nst; ++f) {
-    const double* __restrict__ src = (f < nf) ? yin + f * G.plane : S.dtb_pre + (f - nf) * G.plane;
-    double* dst = tile + f * kTileSize;
-    for (int q = tid; q < kTileSize; q += kTile * kTile) {
-      const int ii = ib + q / kTileW, jj = jb + q % kTileW;
-      if (ii >= lo && ii <= G.Nr && jj >= lo && jj <= G.Nz) dst[q] = src[IDX(G, ii, jj)];
-    }
-  }
-  __syncthreads();
+  stage_tile(G, tile, nst, [&](int f) { return (f < nf) ? yin + f * G.plane : S.dtb_pre + (f - nf) * G.plane; },
+             ib, jb);
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
   if (bi == nbi - 1 && i < 1 + (nbi - 1) * kTile) return;        // done by the tile before
   if (bj == nbj - 1 && j < G.jlo + (nbj - 1) * kTile) return;
-  // the general stencils (with their closure branches) everywhere: a
-  // closure-free interior instantiation schedules more loads early and
-  // spills at two waves per SIMD
-  rhs_point<SHIFT, SCALAR, PHIMODE, false>(G, P, S, TileView{tile, ib, jb, kTileW, kTileSize, nf}, yp, acc, yout,
-                                           i, j);
+  const TileView V{tile, ib, jb, kTileW, kTileSize, nf};
+  // phase 1: the geometric auxiliaries of this point, through gaux (own point
+  // only, so no synchronisation); the memory barrier keeps the compiler from
+  // carrying phase-1 values or loads into phase 2, so the two phases do not
+  // add up in registers
+  if (interior_tile(G, i0, j0))
+    rhs_point<false, false, PHIMODE, true, 1>(G, P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+  else
+    rhs_point<false, false, PHIMODE, false, 1>(G, P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+  asm volatile("" ::: "memory");
+  rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, P, S, V, yp, acc, yout, gaux, i, j);
 }
 
 // ------------------------------------------------------- projection pass
@@ -147,32 +173,38 @@ static int check_launch() {
   return OA_OK;
 }
 
+template <class K>
+static int raise_lds_limit(K* kernel, size_t bytes) {
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          static_cast<int>(bytes)) != hipSuccess) {
+    oa_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed for an RHS kernel");
+    return OA_ERR_HIP;
+  }
+  return OA_OK;
+}
+
 template <bool SH, bool SC, bool PM>
 static int launch_rhs_t(const KGrid& G, const KPhys& P, const KStep& S, const double* yin, const double* yp,
-                        double* acc, double* yout, hipStream_t st) {
+                        double* acc, double* yout, double* gaux, hipStream_t st) {
   // the dynamic LDS of one block exceeds the 64 KiB default; two more
   // planes on a level's first substep (KStep::dtb_pre)
   const size_t per = sizeof(double) * kTileSize;
   const size_t lds = per * static_cast<size_t>(nslots(SH, SC) + (S.dtb_pre ? 2 : 0));
   static bool attr_set = false;
   if (!attr_set) {
-    const size_t most = per * static_cast<size_t>(nslots(SH, SC) + 2);
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rhs<SH, SC, PM>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(most)) != hipSuccess) {
-      oa_set_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed for the RHS kernel");
-      return OA_ERR_HIP;
-    }
+    const int rc = raise_lds_limit(&k_rhs<SH, SC, PM>, per * static_cast<size_t>(nslots(SH, SC) + 2));
+    if (rc != OA_OK) return rc;
     attr_set = true;
   }
   const dim3 blk(kTile, kTile, 1);
   const int nj = G.Nz - G.jlo + 1;
   const dim3 grd(static_cast<unsigned>((nj + kTile - 1) / kTile), static_cast<unsigned>((G.Nr + kTile - 1) / kTile), 1);
-  hipLaunchKernelGGL((k_rhs<SH, SC, PM>), grd, blk, lds, st, G, P, S, yin, yp, acc, yout);
+  hipLaunchKernelGGL((k_rhs<SH, SC, PM>), grd, blk, lds, st, G, P, S, yin, yp, acc, yout, gaux);
   return check_launch();
 }
 
 int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar, bool phimode,
-               const double* yin, const double* yp, double* acc, double* yout, void* stream) {
+               const double* yin, const double* yp, double* acc, double* yout, double* gaux, void* stream) {
   if (G.Nr < kTile || G.Nz - G.jlo + 1 < kTile) {
     oa_set_error("launch_rhs: a level needs at least 16 points in each direction");
     return OA_ERR_ARG;
@@ -180,14 +212,14 @@ int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool 
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int code = (shift ? 4 : 0) + (scalar ? 2 : 0) + (phimode ? 1 : 0);
   switch (code) {
-    case 0: return launch_rhs_t<false, false, false>(G, P, S, yin, yp, acc, yout, st);
-    case 1: return launch_rhs_t<false, false, true>(G, P, S, yin, yp, acc, yout, st);
-    case 2: return launch_rhs_t<false, true, false>(G, P, S, yin, yp, acc, yout, st);
-    case 3: return launch_rhs_t<false, true, true>(G, P, S, yin, yp, acc, yout, st);
-    case 4: return launch_rhs_t<true, false, false>(G, P, S, yin, yp, acc, yout, st);
-    case 5: return launch_rhs_t<true, false, true>(G, P, S, yin, yp, acc, yout, st);
-    case 6: return launch_rhs_t<true, true, false>(G, P, S, yin, yp, acc, yout, st);
-    default: return launch_rhs_t<true, true, true>(G, P, S, yin, yp, acc, yout, st);
+    case 0: return launch_rhs_t<false, false, false>(G, P, S, yin, yp, acc, yout, gaux, st);
+    case 1: return launch_rhs_t<false, false, true>(G, P, S, yin, yp, acc, yout, gaux, st);
+    case 2: return launch_rhs_t<false, true, false>(G, P, S, yin, yp, acc, yout, gaux, st);
+    case 3: return launch_rhs_t<false, true, true>(G, P, S, yin, yp, acc, yout, gaux, st);
+    case 4: return launch_rhs_t<true, false, false>(G, P, S, yin, yp, acc, yout, gaux, st);
+    case 5: return launch_rhs_t<true, false, true>(G, P, S, yin, yp, acc, yout, gaux, st);
+    case 6: return launch_rhs_t<true, true, false>(G, P, S, yin, yp, acc, yout, gaux, st);
+    default: return launch_rhs_t<true, true, true>(G, P, S, yin, yp, acc, yout, gaux, st);
   }
 }
 
