@@ -86,6 +86,10 @@ struct KTime {
   double c0, c1, c2;    // border 2: coarse value, strip t0, strip tm1
 };
 
+struct FbPoint;   // one fine-boundary point with its coarse cell (oaxis_device.h)
+long fb_points(const KGrid& Gf, int depth);   // records of a zone of that depth
+size_t fb_record_bytes();                     // sizeof(FbPoint)
+
 // launchers (oaxis_kernels.hip)
 // One RK substep of a level (all its points); gaux holds NAUX planes of
 // per-point geometric auxiliaries passed between the two RHS kernels.
@@ -98,11 +102,12 @@ int launch_deldef(const KGrid& G, double* y, void* stream);
 int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream);
 int launch_saveold(const KGrid& G, int nf, const double* y, double* strips_rR_t0,
                    double* strips_zL_t0, double* strips_zR_t0, void* stream);
-int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T,
-                     bool shift, bool scalar, bool time0, const double* ycoarse,
-                     double* yfine, const double* rR0, const double* rR1,
-                     const double* zL0, const double* zL1, const double* zR0,
-                     const double* zR1, int interporder, void* stream);
+int launch_fb_table(const KGrid& Gf, const KGrid& Gc, int depth, int interporder, FbPoint* table, long n,
+                    void* stream);
+int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, bool shift, bool scalar,
+                     bool time0, const FbPoint* table, long n, const double* yc, double* yf, const double* rR0,
+                     const double* rR1, const double* zL0, const double* zL1, const double* zR0, const double* zR1,
+                     void* stream);
 int launch_restrict(const KGrid& Gf, const KGrid& Gc, int nf, int interporder,
                     const double* yfine, double* ycoarse, void* stream);
 int launch_transpose_in(const KGrid& G, const double* host_colmajor_dev, int ld_src,

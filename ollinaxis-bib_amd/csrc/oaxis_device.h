@@ -11,6 +11,10 @@
 #ifndef OA_HD
 #define OA_HD inline
 #endif
+// functions the host side calls too
+#ifndef OA_HHD
+#define OA_HHD inline
+#endif
 
 namespace oax {
 
@@ -1026,23 +1030,32 @@ OA_HD double apply(const Cell& c, const double* __restrict__ f, const KGrid& G) 
   return out;
 }
 
-// finebound(box 0, level, dtw, .true.): one thread per fine strip point.
+// finebound(box 0, level, dtw, .true.).  The fine boundary points and the
+// coarse cells they interpolate from are fixed per level and zone depth, so
+// fb_geometry resolves them once into FbPoint records; finebound_point then
+// does the per-call work (one thread per (point, field group)).
 // Point classes: upper-r strip rows not claimed by a z strip, then the upper
 // and (without equatorial symmetry) lower z strips over the whole r range,
 // which is also the reference's overwrite order at the corners.
-template <bool SHIFT, bool SCALAR>
-OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, int time0,
-                           int interporder, const double* __restrict__ yc, double* __restrict__ yf,
-                           const double* __restrict__ rR0, const double* __restrict__ rR1,
-                           const double* __restrict__ zL0, const double* __restrict__ zL1,
-                           const double* __restrict__ zR0, const double* __restrict__ zR1, long q, int grp) {
-  const int g = Gf.g, D = T.depth;
-  const int nrt = Gf.Nr + g, nzt = Gf.Nz + g;
+struct FbPoint {
+  Cell c;            // coarse cell and interpolation weights
+  int i, j;          // fine point
+  int which;         // strip: 0 upper r, 1 upper z, 2 lower z; -1: no point
+  int sidx;          // index of the point in its strip
+  int strip_ok;      // the point lies in the saved strips (depth < ghost)
+};
+
+// number of records for zone depth D (last layer index)
+OA_HHD long fb_count(const KGrid& Gf, int D) {
   const int nz_strips = Gf.eqsym ? 1 : 2;
-  const int nzrow = (D + 1) * nz_strips;                 // rows claimed by z strips
+  return static_cast<long>(D + 1) * (Gf.Nz + Gf.g) + static_cast<long>(D + 1) * nz_strips * (Gf.Nr + Gf.g);
+}
+
+OA_HD void fb_geometry(const KGrid& Gf, const KGrid& Gc, int D, int interporder, long q, FbPoint& fp) {
+  const int g = Gf.g;
+  const int nrt = Gf.Nr + g, nzt = Gf.Nz + g;
   const long n_r = static_cast<long>(D + 1) * nzt;        // upper-r strip (rows filtered below)
-  const long n_z = static_cast<long>(nzrow) * nrt;
-  if (q >= n_r + n_z) return;
+  fp.which = -1;
   int i, j, m = 0, n = 0, which;   // which: 0 rR, 1 zR, 2 zL
   if (q < n_r) {
     m = static_cast<int>(q / nzt);
@@ -1059,7 +1072,7 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
     if (row <= D) { n = row; j = Gf.Nz - n; which = 1; }
     else { n = row - (D + 1); j = 1 - g + n; which = 2; }
   }
-  if (i < 1) return;   // axis images are refreshed by parity below
+  if (i < 1) return;   // axis images are refreshed by parity
   // coordinates of the fine point as finebound.f90 forms them
   const double rminf = coord_r(Gf, 1 - g), rmaxf = coord_r(Gf, Gf.Nr);
   const double zminf = coord_z(Gf, 1 - g), zmaxf = coord_z(Gf, Gf.Nz);
@@ -1067,17 +1080,31 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
   if (which == 0) { r0 = rmaxf - static_cast<double>(m) * Gf.dr; z0 = zminf + static_cast<double>(j + g - 1) * Gf.dz; }
   else if (which == 1) { r0 = rminf + static_cast<double>(i + g - 1) * Gf.dr; z0 = zmaxf - static_cast<double>(n) * Gf.dz; }
   else { r0 = rminf + static_cast<double>(i + g - 1) * Gf.dr; z0 = zminf + static_cast<double>(n) * Gf.dz; }
-  const Cell c = locate(Gc, r0, z0, interporder);
+  fp.c = locate(Gc, r0, z0, interporder);
+  fp.i = i;
+  fp.j = j;
+  fp.which = which;
+  fp.sidx = (which == 0) ? m * nzt + (j + g - 1) : n * nrt + (i + g - 1);
+  fp.strip_ok = ((which == 0 ? m : n) < g) ? 1 : 0;
+}
+
+template <bool SHIFT, bool SCALAR>
+OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, int time0,
+                           const double* __restrict__ yc, double* __restrict__ yf,
+                           const double* __restrict__ rR0, const double* __restrict__ rR1,
+                           const double* __restrict__ zL0, const double* __restrict__ zL1,
+                           const double* __restrict__ zR0, const double* __restrict__ zR1, const FbPoint& fp,
+                           int grp) {
+  if (fp.which < 0) return;
+  const int g = Gf.g, i = fp.i, j = fp.j;
+  const Cell& c = fp.c;
   const long pf = Gf.plane, pc = Gc.plane;
-  const long k = IDX(Gf, i, j);
-  // strip index of this point for the time levels
-  const double* s0 = nullptr;
-  const double* s1 = nullptr;
-  long sidx = 0, sper = 0;
-  if (which == 0) { s0 = rR0; s1 = rR1; sidx = static_cast<long>(m) * nzt + (j + g - 1); sper = static_cast<long>(g) * nzt; }
-  else if (which == 1) { s0 = zR0; s1 = zR1; sidx = static_cast<long>(n) * nrt + (i + g - 1); sper = static_cast<long>(g) * nrt; }
-  else { s0 = zL0; s1 = zL1; sidx = static_cast<long>(n) * nrt + (i + g - 1); sper = static_cast<long>(g) * nrt; }
-  const bool strip_ok = (which == 0 ? m : n) < g;
+  // the strip this point belongs to, for the time levels
+  const double* s0 = (fp.which == 0) ? rR0 : (fp.which == 1) ? zR0 : zL0;
+  const double* s1 = (fp.which == 0) ? rR1 : (fp.which == 1) ? zR1 : zL1;
+  const long sper = static_cast<long>(g) * ((fp.which == 0) ? (Gf.Nz + g) : (Gf.Nr + g));
+  const long sidx = fp.sidx;
+  const bool strip_ok = fp.strip_ok != 0;
 
   // time interpolation of one injected value (bound_interp.inc)
   auto value = [&](int s) -> double {
@@ -1105,7 +1132,6 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
     slot_parity(SHIFT, SCALAR, s, symr, symz);
     put(yf + s * pf, Gf, i, j, value(s), symr, symz);
   }
-  (void)k;
 }
 
 // groups of finebound_point: the projected set, then one per remaining slot

@@ -38,6 +38,8 @@ struct Level {
   double t = 0.0, t1 = 0.0, t2 = 0.0;
   int s = 0;
   long strip_r = 0, strip_z = 0;                    // doubles per strip set piece
+  FbPoint* fbtab[2] = {nullptr, nullptr};           // fine-boundary geometry: depth g+2, depth g-1
+  long fbn[2] = {0, 0};
 };
 
 }  // namespace oax
@@ -231,9 +233,10 @@ int do_finebound(oa_handle* h, int level, double dtw) {
   T.c2 = (tl - tp) * (tl - t0) / ((tm1 - tp) * (tm1 - t0));
   const bool time0 = (L.t == 0.0);
   double* yf = L.slab[L.in_step ? L.last_out : L.cur];
-  RC(launch_finebound(L.G, Lc.G, h->phys, T, h->shift, h->scalar, time0, Lc.slab[Lc.cur], yf,
-                      strip_ptr(h, L, 0, 0), strip_ptr(h, L, 1, 0), strip_ptr(h, L, 0, 1), strip_ptr(h, L, 1, 1),
-                      strip_ptr(h, L, 0, 2), strip_ptr(h, L, 1, 2), h->p.interporder, h->stream));
+  const int tab = T.same ? 0 : 1;
+  RC(launch_finebound(L.G, Lc.G, h->phys, T, h->shift, h->scalar, time0, L.fbtab[tab], L.fbn[tab], Lc.slab[Lc.cur],
+                      yf, strip_ptr(h, L, 0, 0), strip_ptr(h, L, 1, 0), strip_ptr(h, L, 0, 1),
+                      strip_ptr(h, L, 1, 1), strip_ptr(h, L, 0, 2), strip_ptr(h, L, 1, 2), h->stream));
   return OA_OK;
 }
 
@@ -370,6 +373,15 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
     for (int b = 0; b < 3; ++b) ok = ok && hipMalloc(&L.strip[b], sbytes) == hipSuccess;
     if (h->shift) ok = ok && hipMalloc(&L.dtb_pre, sizeof(double) * 2 * static_cast<size_t>(L.G.plane)) == hipSuccess;
     ok = ok && hipMalloc(&L.gaux, sizeof(double) * NAUX * static_cast<size_t>(L.G.plane)) == hipSuccess;
+    if (l > 0) {
+      const int depth[2] = {L.G.g + 2, L.G.g - 1};
+      for (int k = 0; k < 2; ++k) {
+        L.fbn[k] = fb_points(L.G, depth[k]);
+        ok = ok && hipMalloc(reinterpret_cast<void**>(&L.fbtab[k]), fb_record_bytes() * L.fbn[k]) == hipSuccess;
+        ok = ok && launch_fb_table(L.G, h->L[l - 1].G, depth[k], p.interporder, L.fbtab[k], L.fbn[k], h->stream) ==
+                       OA_OK;
+      }
+    }
     if (!ok) {
       oa_destroy(h);
       oa_set_error("device allocation failed");
@@ -401,6 +413,8 @@ extern "C" int oa_destroy(oa_handle* h) {
     if (L.acc) (void)hipFree(L.acc);
     if (L.dtb_pre) (void)hipFree(L.dtb_pre);
     if (L.gaux) (void)hipFree(L.gaux);
+    for (int k = 0; k < 2; ++k)
+      if (L.fbtab[k]) (void)hipFree(L.fbtab[k]);
   }
   for (hipEvent_t e : h->kev) (void)hipEventDestroy(e);
   if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);

@@ -21,6 +21,7 @@
 #include "oa_internal.h"
 
 #define OA_HD __device__ __forceinline__
+#define OA_HHD __host__ __device__ inline
 #include "oaxis_device.h"
 
 namespace oax {
@@ -139,14 +140,26 @@ __global__ void k_saveold(KGrid G, int nf, const double* __restrict__ y, double*
 
 // finebound(box 0, level, dtw, .true.): one thread per fine strip point
 template <bool SHIFT, bool SCALAR>
-__global__ void __launch_bounds__(256) k_finebound(KGrid Gf, KGrid Gc, KPhys P, KTime T, int time0, int interporder,
+__global__ void __launch_bounds__(256) k_finebound(KGrid Gf, KGrid Gc, KPhys P, KTime T, int time0,
+                                                   const FbPoint* __restrict__ table, long n,
                                                    const double* __restrict__ yc, double* __restrict__ yf,
                                                    const double* __restrict__ rR0, const double* __restrict__ rR1,
                                                    const double* __restrict__ zL0, const double* __restrict__ zL1,
                                                    const double* __restrict__ zR0, const double* __restrict__ zR1) {
   const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  finebound_point<SHIFT, SCALAR>(Gf, Gc, P, T, time0, interporder, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1, q,
+  if (q >= n) return;
+  finebound_point<SHIFT, SCALAR>(Gf, Gc, P, T, time0, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1, table[q],
                                  static_cast<int>(blockIdx.y));
+}
+
+// the fixed geometry of the fine boundary zone of depth D (once per level)
+__global__ void __launch_bounds__(256) k_fb_table(KGrid Gf, KGrid Gc, int D, int interporder, long n,
+                                                  FbPoint* __restrict__ table) {
+  const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  FbPoint fp;
+  fb_geometry(Gf, Gc, D, interporder, q, fp);
+  table[q] = fp;
 }
 
 // restrict(box 0, level, .true.): one thread per (coarse point, field)
@@ -261,21 +274,27 @@ int launch_saveold(const KGrid& G, int nf, const double* y, double* rR, double* 
   return check_launch();
 }
 
+long fb_points(const KGrid& Gf, int depth) { return fb_count(Gf, depth); }
+size_t fb_record_bytes() { return sizeof(FbPoint); }
+
+int launch_fb_table(const KGrid& Gf, const KGrid& Gc, int depth, int interporder, FbPoint* table, long n,
+                    void* stream) {
+  hipLaunchKernelGGL(k_fb_table, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), Gf, Gc, depth, interporder, n, table);
+  return check_launch();
+}
+
 int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, bool shift, bool scalar,
-                     bool time0, const double* yc, double* yf, const double* rR0, const double* rR1,
-                     const double* zL0, const double* zL1, const double* zR0, const double* zR1, int interporder,
+                     bool time0, const FbPoint* table, long n, const double* yc, double* yf, const double* rR0,
+                     const double* rR1, const double* zL0, const double* zL1, const double* zR0, const double* zR1,
                      void* stream) {
-  const int g = Gf.g, D = T.depth;
-  const long n_r = static_cast<long>(D + 1) * (Gf.Nz + g);
-  const long n_z = static_cast<long>(D + 1) * (Gf.eqsym ? 1 : 2) * (Gf.Nr + g);
-  const long n = n_r + n_z;
   const int blk = 256;
-  const dim3 grd(static_cast<unsigned>((n + blk - 1) / blk),
-                 static_cast<unsigned>(finebound_groups(shift, scalar)), 1);
+  const dim3 grd(static_cast<unsigned>((n + blk - 1) / blk), static_cast<unsigned>(finebound_groups(shift, scalar)), 1);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int t0 = time0 ? 1 : 0;
 #define LAUNCH_FB(SH, SC) \
-  hipLaunchKernelGGL((k_finebound<SH, SC>), grd, dim3(blk), 0, st, Gf, Gc, P, T, t0, interporder, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1)
+  hipLaunchKernelGGL((k_finebound<SH, SC>), grd, dim3(blk), 0, st, Gf, Gc, P, T, t0, table, n, yc, yf, rR0, rR1, \
+                     zL0, zL1, zR0, zR1)
   if (shift && scalar) LAUNCH_FB(true, true);
   else if (shift) LAUNCH_FB(true, false);
   else if (scalar) LAUNCH_FB(false, true);
