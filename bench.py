@@ -47,6 +47,21 @@ def bytes_per_update(params, nfields):
     return 8.0 * nfields * (2 + 3 * (it - 1)), it
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_rhs launch for this workload from the committed
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "k_rhs_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(text, steps):
     """The oracle (plain C restatement, one core) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -66,6 +81,20 @@ def cpu_baseline(text, steps):
     return per_coarse * steps / dt, dt
 
 
+def copy_probe(nbytes):
+    """A plain device-to-device copy of nbytes, three times: the calibration
+    kernel for the FETCH_SIZE / WRITE_SIZE counters.  Runs in a process of its
+    own (torch's HIP runtime and the library's must not share one)."""
+    import torch
+    n = nbytes // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +104,12 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (every level of a spacetime stays resident on "
@@ -133,7 +167,7 @@ def main():
                    "integrator": "rk4" if params.integrator == 1 else "icn",
                    "point_updates": updates, "device_ms": device_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_rhs", "avg_launch_ms": rhs_ms, "launches": rhs_n,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
