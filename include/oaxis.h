@@ -209,9 +209,10 @@ int oa_sync(oa_handle* h);
 
 /* Closed-form initial data on every level (host side, not the hot path):
  * minkowski (+ gauss lapse/shift perturbations), schwarzschild,
- * BrillLindquist, scalarpulse with psi = 1 (no Hamiltonian-constraint
- * solve).  Follows initial.f90:53-150,189-330,528-620 and
- * idata_scalarpulse.f90:60-103. */
+ * BrillLindquist, BrillWave (Holz) and scalarpulse, the last two with
+ * psi = 1 (no Hamiltonian-constraint solve).  Follows
+ * initial.f90:53-150,189-330,528-620, idata_brillwave.f90:88-105,183-208
+ * and idata_scalarpulse.f90:60-103. */
 int oa_initial_data(oa_handle* h);
 
 /* Device time of the last oa_evolve / oa_coarse_step call in ms (HIP events
@@ -226,6 +227,43 @@ int oa_rhs_kernel_stats(const oa_handle* h, double* avg_ms, long* launches);
 /* Grid-point updates (sum over levels and steps of Nrtotal*Nztotal) done
  * by this handle so far. */
 int oa_point_updates(const oa_handle* h, double* updates);
+
+/* ---- batches of spacetimes
+ * No counterpart in the reference, which evolves one spacetime per run.
+ * For callers that evolve many independent spacetimes at once (parameter
+ * scans, the serving loop): every member advances by one coarse step per
+ * oa_batch_coarse_step, and each kernel of the step runs as ONE launch over
+ * the work items (spacetime, level) of all members that take part in it.
+ * Members may differ in grid size, level count, integrator and physics.
+ * Each member's arithmetic is that of oa_coarse_step on the member alone:
+ * results are bit-identical.  While a member, a handle's own calls
+ * (download, upload, ...) run on the batch's stream, in order with the
+ * batch's steps; oa_coarse_step / oa_evolve on a member are refused. */
+typedef struct oa_batch oa_batch;
+int oa_batch_create(oa_batch** out);
+int oa_batch_destroy(oa_batch* b);          /* members leave first; none is destroyed */
+/* h must have its state projected (oa_initial_aux done); it joins at its
+ * current level clocks. */
+int oa_batch_add(oa_batch* b, oa_handle* h);
+int oa_batch_remove(oa_batch* b, oa_handle* h);
+int oa_batch_size(const oa_batch* b, int* n);
+/* One coarse step of every member (onestep(0) of each, level by level in
+ * lockstep).  Returns after the work is enqueued; oa_batch_sync waits. */
+int oa_batch_coarse_step(oa_batch* b);
+int oa_batch_evolve(oa_batch* b, int nsteps);
+int oa_batch_sync(oa_batch* b);
+/* Device time (ms) of the last oa_batch_evolve, from HIP events. */
+int oa_batch_last_elapsed_ms(const oa_batch* b, double* ms);
+/* Grid-point updates by all members during this batch's steps. */
+int oa_batch_point_updates(const oa_batch* b, double* updates);
+/* Launches enqueued by the last oa_batch_coarse_step (one per kernel kind
+ * and physics group) and the work items they covered. */
+int oa_batch_launch_stats(const oa_batch* b, long* launches, long* items);
+/* HIP-event timing of every RHS launch during oa_batch_evolve (off by
+ * default): total kernel time (ms), launches, and grid points (Nr x Nz of
+ * each item) those launches advanced by one substep. */
+int oa_batch_set_kernel_timing(oa_batch* b, int on);
+int oa_batch_rhs_kernel_stats(const oa_batch* b, double* total_ms, long* launches, double* points);
 
 #ifdef __cplusplus
 }

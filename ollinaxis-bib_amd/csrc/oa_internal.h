@@ -90,27 +90,64 @@ struct FbPoint;   // one fine-boundary point with its coarse cell (oaxis_device.
 long fb_points(const KGrid& Gf, int depth);   // records of a zone of that depth
 size_t fb_record_bytes();                     // sizeof(FbPoint)
 
-// launchers (oaxis_kernels.hip)
-// One RK substep of a level (all its points); gaux holds NAUX planes of
-// per-point geometric auxiliaries passed between the two RHS kernels.
-int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar,
-               bool phimode, const double* yin, const double* yp, double* acc, double* yout,
-               double* gaux, void* stream);
-int launch_project(const KGrid& G, const KPhys& P, bool shift, bool scalar, bool time0,
-                   double* y, void* stream);
-int launch_deldef(const KGrid& G, double* y, void* stream);
+// ------------------------------------------------------------ work items
+// Each kernel of a substep runs over a table of work items in device memory,
+// one item per (spacetime, level): everything one level's share of the
+// launch needs.  Item k owns the blocks [block0, block0 + blocks) of the
+// grid.  A spacetime alone launches tables of one item; a batch of
+// spacetimes launches one table over all its members.
+struct RhsItem {                  // one RK substep of a level
+  KGrid G; KPhys P; KStep S;
+  const double* yin; const double* yp; double* acc; double* yout; double* gaux;
+  int nbi, nbj;                   // 16 x 16 tiles along r and z
+  int block0;
+};
+struct ProjItem {                 // state projection (chi/phi, factordet) of a level
+  KGrid G; KPhys P; int time0; double* y;
+  int block0;
+};
+struct DelItem {                  // Delta = DelDef on a level at clock 0
+  KGrid G; double* y;
+  int block0;
+};
+struct SaveItem {                 // saveold of the fine-boundary strips
+  KGrid G; int nf; const double* y; double* rR; double* zL; double* zR;
+  int block0;
+};
+struct FbItem {                   // finebound of a fine level
+  KGrid Gf, Gc; KPhys P; KTime T; int time0;
+  const FbPoint* table; long n;   // the zone's point records
+  const double* yc; double* yf;
+  const double* rR0; const double* rR1; const double* zL0; const double* zL1; const double* zR0; const double* zR1;
+  int block0;
+};
+struct RsItem {                   // restriction of a fine level onto its parent
+  KGrid Gf, Gc; int nf, interporder; const double* yf; double* yc;
+  int block0;
+};
+
+// blocks one item occupies in a launch (host side, for block0 prefix sums)
+int rhs_blocks(const RhsItem& it);
+int proj_blocks(const ProjItem& it);
+int del_blocks(const DelItem& it);
+int save_blocks(const SaveItem& it);
+int fb_blocks(const FbItem& it, bool shift, bool scalar);
+int rs_blocks(const RsItem& it);
+// tiles of a level for the RHS kernel; OA_ERR_ARG if a direction has fewer
+// than 16 points
+int rhs_tiles(const KGrid& G, int* nbi, int* nbj);
+
+// launchers (oaxis_kernels.hip): n items in device memory whose block0
+// fields are the prefix sums of their blocks, and the total
+int launch_rhs_table(const RhsItem* items, int n, int blocks, bool shift, bool scalar, bool phimode,
+                     bool any_dtb_pre, void* stream);
+int launch_project_table(const ProjItem* items, int n, int blocks, bool shift, bool scalar, void* stream);
+int launch_deldef_table(const DelItem* items, int n, int blocks, void* stream);
+int launch_saveold_table(const SaveItem* items, int n, int blocks, void* stream);
+int launch_finebound_table(const FbItem* items, int n, int blocks, bool shift, bool scalar, void* stream);
+int launch_restrict_table(const RsItem* items, int n, int blocks, void* stream);
 int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream);
-int launch_saveold(const KGrid& G, int nf, const double* y, double* strips_rR_t0,
-                   double* strips_zL_t0, double* strips_zR_t0, void* stream);
 int launch_fb_table(const KGrid& Gf, const KGrid& Gc, int depth, int interporder, FbPoint* table, long n,
                     void* stream);
-int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, bool shift, bool scalar,
-                     bool time0, const FbPoint* table, long n, const double* yc, double* yf, const double* rR0,
-                     const double* rR1, const double* zL0, const double* zL1, const double* zR0, const double* zR1,
-                     void* stream);
-int launch_restrict(const KGrid& Gf, const KGrid& Gc, int nf, int interporder,
-                    const double* yfine, double* ycoarse, void* stream);
-int launch_transpose_in(const KGrid& G, const double* host_colmajor_dev, int ld_src,
-                        double* plane, int sym_r, int sym_z, void* stream);
 
 }  // namespace oax

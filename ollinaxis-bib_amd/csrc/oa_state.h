@@ -1,0 +1,81 @@
+// Host-side state of a spacetime handle and the pieces of one Berger-Oliger
+// step, shared by the single-spacetime runtime and the batch runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "oa_internal.h"
+#include "oa_plan.h"
+
+namespace oax {
+
+struct Level {
+  KGrid G{};
+  double* slab[3] = {nullptr, nullptr, nullptr};   // state buffers
+  double* acc = nullptr;                            // RK4 accumulator
+  double* strip[3] = {nullptr, nullptr, nullptr};  // fine-boundary time levels (ring)
+  int ring[3] = {0, 1, 2};                          // ring[k]: strip set of time level -k
+  int cur = 0;                                      // slab holding the current state
+  int step_a = 0, last_out = 0;                     // roles inside a step
+  bool in_step = false;
+  bool dirty = true;                                // state needs the full projection
+  bool phi_valid = true;                            // phi plane holds the reference's phi
+  bool aux_phi = true;                              // last auxiliary ran at clock 0 (phi mode)
+  double* gaux = nullptr;                           // geometric auxiliaries between the two RHS phases
+  double* dtb_pre = nullptr;                        // dtbeta_r, dtbeta_z before the driver initialisation
+  bool dtb_pre_valid = false;                       // the next substep differentiates dtb_pre
+  double t = 0.0, t1 = 0.0, t2 = 0.0;
+  int s = 0;
+  long strip_r = 0, strip_z = 0;                    // doubles per strip set piece
+  FbPoint* fbtab[2] = {nullptr, nullptr};           // fine-boundary geometry: depth g+2, depth g-1
+  long fbn[2] = {0, 0};
+};
+
+}  // namespace oax
+
+struct oa_batch;
+
+struct oa_handle {
+  oa_params p{};
+  int nlev = 0;
+  int nf = 0;
+  bool shift = false, scalar = false;
+  double dt0 = 0.0;
+  std::vector<oax::Level> L;
+  hipStream_t stream = nullptr;      // the stream every call of this handle uses
+  hipStream_t own_stream = nullptr;  // its own; a batch lends its stream while the handle is a member
+  oa_batch* batch = nullptr;
+  oax::KPhys phys{};
+  oax::Plan plan;                    // launches of this handle's own calls
+  // timing
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  double last_ms = 0.0;
+  double rhs_avg_ms = 0.0;
+  long rhs_launches = 0;
+  double point_updates = 0.0;
+};
+
+namespace oax {
+
+// The pieces of onestep.f90 for one level, split into the bookkeeping that
+// fills a work item (prepare_*) and its launch, so that a batch can gather
+// the items of many spacetimes into one launch.  Each prepare_* advances the
+// handle's bookkeeping exactly as the launch it describes will advance the
+// device state.
+int niter_of(const oa_handle* h);
+void substep_weights(const oa_handle* h, int level, int iter, double* dtw, double* w);
+int prepare_saveold(oa_handle* h, int level, SaveItem* it);
+int prepare_substep(oa_handle* h, int level, int iter, int niter, double dtw, double w, RhsItem* it,
+                    bool* phimode);
+bool prepare_finebound(oa_handle* h, int level, double dtw, FbItem* it);   // false on level 0
+// auxiliary(level): *proj / *del say whether the projection / DelDef run
+void prepare_auxiliary(oa_handle* h, int level, ProjItem* pj, bool* proj, DelItem* dd, bool* del);
+void prepare_restrict(oa_handle* h, int level, RsItem* it);
+void finish_level_step(oa_handle* h, int level);   // advance the level clock, count point updates
+// onestep(level) of every spacetime of `all` that has the level, in lockstep,
+// appended to `plan` (oa_plan.cpp)
+int plan_level(Plan& plan, const std::vector<oa_handle*>& all, int level);
+
+}  // namespace oax

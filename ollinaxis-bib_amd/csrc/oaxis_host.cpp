@@ -18,53 +18,9 @@ thread_local std::string g_last_error;
 void oa_set_error(const std::string& msg) { g_last_error = msg; }
 extern "C" const char* oa_last_error(void) { return g_last_error.c_str(); }
 
-namespace oax {
-
-struct Level {
-  KGrid G{};
-  double* slab[3] = {nullptr, nullptr, nullptr};   // state buffers
-  double* acc = nullptr;                            // RK4 accumulator
-  double* strip[3] = {nullptr, nullptr, nullptr};  // fine-boundary time levels (ring)
-  int ring[3] = {0, 1, 2};                          // ring[k]: strip set of time level -k
-  int cur = 0;                                      // slab holding the current state
-  int step_a = 0, last_out = 0;                     // roles inside a step
-  bool in_step = false;
-  bool dirty = true;                                // state needs the full projection
-  bool phi_valid = true;                            // phi plane holds the reference's phi
-  bool aux_phi = true;                              // last auxiliary ran at clock 0 (phi mode)
-  double* gaux = nullptr;                           // geometric auxiliaries between the two RHS kernels
-  double* dtb_pre = nullptr;                        // dtbeta_r, dtbeta_z before the driver initialisation
-  bool dtb_pre_valid = false;                       // the next substep differentiates dtb_pre
-  double t = 0.0, t1 = 0.0, t2 = 0.0;
-  int s = 0;
-  long strip_r = 0, strip_z = 0;                    // doubles per strip set piece
-  FbPoint* fbtab[2] = {nullptr, nullptr};           // fine-boundary geometry: depth g+2, depth g-1
-  long fbn[2] = {0, 0};
-};
-
-}  // namespace oax
+#include "oa_state.h"
 
 using namespace oax;
-
-struct oa_handle {
-  oa_params p{};
-  int nlev = 0;
-  int nf = 0;
-  bool shift = false, scalar = false;
-  double dt0 = 0.0;
-  std::vector<Level> L;
-  hipStream_t stream = nullptr;
-  KPhys phys{};
-  // timing
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  double last_ms = 0.0;
-  bool kernel_timing = false;
-  std::vector<hipEvent_t> kev;     // pairs around k_rhs launches
-  size_t kev_used = 0;
-  double rhs_avg_ms = 0.0;
-  long rhs_launches = 0;
-  double point_updates = 0.0;
-};
 
 namespace {
 
@@ -148,23 +104,51 @@ double* strip_ptr(oa_handle* h, Level& L, int tl, int piece) {
   return base + nf * L.strip_r + nf * L.strip_z;
 }
 
+}  // namespace
+
 // ------------------------------------------------------------ step pieces
-int do_saveold(oa_handle* h, int level) {
+namespace oax {
+
+int niter_of(const oa_handle* h) { return (h->p.integrator == OA_RK4) ? 4 : h->p.icniter; }
+
+// dtw and the RK4 weight of internal iteration iter (onestep.f90:141-177)
+void substep_weights(const oa_handle* h, int level, int iter, double* dtw, double* w) {
+  const double dt = h->L[level].G.dt;
+  *w = 0.0;
+  if (h->p.integrator == OA_RK4) {
+    switch (iter) {
+      case 1: *dtw = 0.5 * dt; *w = 1.0 / 6.0; break;
+      case 2: *dtw = 0.5 * dt; *w = 1.0 / 3.0; break;
+      case 3: *dtw = dt; *w = 1.0 / 3.0; break;
+      default: *dtw = dt; *w = 1.0 / 6.0; break;
+    }
+  } else {
+    *dtw = (iter < h->p.icniter) ? 0.5 * dt : dt;
+  }
+}
+
+int prepare_saveold(oa_handle* h, int level, SaveItem* it) {
   Level& L = h->L[level];
   // time levels shift: t-2 <- t-1 <- t0 <- new copy
   const int oldest = L.ring[2];
   L.ring[2] = L.ring[1];
   L.ring[1] = L.ring[0];
   L.ring[0] = oldest;
-  RC(launch_saveold(L.G, h->nf, L.slab[L.cur], strip_ptr(h, L, 0, 0), strip_ptr(h, L, 0, 1),
-                    strip_ptr(h, L, 0, 2), h->stream));
+  *it = SaveItem{};
+  it->G = L.G;
+  it->nf = h->nf;
+  it->y = L.slab[L.cur];
+  it->rR = strip_ptr(h, L, 0, 0);
+  it->zL = strip_ptr(h, L, 0, 1);
+  it->zR = strip_ptr(h, L, 0, 2);
   L.step_a = L.cur;
   L.last_out = L.cur;
   L.in_step = true;
   return OA_OK;
 }
 
-int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double w) {
+int prepare_substep(oa_handle* h, int level, int iter, int niter, double dtw, double w, RhsItem* it,
+                    bool* phimode) {
   Level& L = h->L[level];
   if (!L.in_step) {
     oa_set_error("oa_substep called before oa_saveold");
@@ -174,6 +158,8 @@ int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double 
     oa_set_error("oa_substep on a level whose state is not projected (call oa_auxiliary / oa_initial_aux)");
     return OA_ERR_STATE;
   }
+  *it = RhsItem{};
+  RC(rhs_tiles(L.G, &it->nbi, &it->nbj));
   const int a = L.step_a;
   const int in = (iter == 1) ? a : L.last_out;
   int out;
@@ -184,29 +170,19 @@ int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double 
     while (out == a || out == in) ++out;
   }
   const bool time0 = (L.t == 0.0);
-  const bool phimode = L.aux_phi;
-  KPhys P = h->phys;
-  P.radiative = (level == 0 && h->p.boundtype == OA_BOUND_RADIATIVE) ? 1 : 0;
-  KStep S{iter, niter, h->p.integrator == OA_RK4 ? 1 : 0, dtw, w, time0 ? 1 : 0,
-          L.dtb_pre_valid ? L.dtb_pre : nullptr};
-  if (h->kernel_timing) {
-    if (h->kev_used + 2 > h->kev.size()) {
-      for (int q = 0; q < 256; ++q) {
-        hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
-        h->kev.push_back(e);
-      }
-    }
-    HIPCHK(hipEventRecord(h->kev[h->kev_used], h->stream));
-  }
-  RC(launch_rhs(L.G, P, S, h->shift, h->scalar, phimode, L.slab[in], L.slab[a], L.acc, L.slab[out], L.gaux,
-                h->stream));
+  *phimode = L.aux_phi;
+  it->G = L.G;
+  it->P = h->phys;
+  it->P.radiative = (level == 0 && h->p.boundtype == OA_BOUND_RADIATIVE) ? 1 : 0;
+  it->S = KStep{iter, niter, h->p.integrator == OA_RK4 ? 1 : 0, dtw, w, time0 ? 1 : 0,
+                L.dtb_pre_valid ? L.dtb_pre : nullptr};
+  it->yin = L.slab[in];
+  it->yp = L.slab[a];
+  it->acc = L.acc;
+  it->yout = L.slab[out];
+  it->gaux = L.gaux;
   L.aux_phi = time0;
   L.dtb_pre_valid = false;
-  if (h->kernel_timing) {
-    HIPCHK(hipEventRecord(h->kev[h->kev_used + 1], h->stream));
-    h->kev_used += 2;
-  }
   L.last_out = out;
   if (!time0 && h->p.chimethod) L.phi_valid = false;
   if (iter == niter) {
@@ -216,12 +192,13 @@ int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double 
   return OA_OK;
 }
 
-int do_finebound(oa_handle* h, int level, double dtw) {
-  if (level == 0) return OA_OK;
+bool prepare_finebound(oa_handle* h, int level, double dtw, FbItem* it) {
+  if (level == 0) return false;
   Level& L = h->L[level];
   Level& Lc = h->L[level - 1];
   const double tp = Lc.t, t0 = L.t, tm1 = L.t1, tl = t0 + dtw;
-  KTime T{};
+  *it = FbItem{};
+  KTime& T = it->T;
   T.same = (tp == tl) ? 1 : 0;
   T.depth = T.same ? L.G.g + 2 : L.G.g - 1;
   T.border = (tm1 == 0.0) ? 1 : 2;
@@ -231,79 +208,125 @@ int do_finebound(oa_handle* h, int level, double dtw) {
   T.c0 = (tl - t0) * (tl - tm1) / ((tp - t0) * (tp - tm1));
   T.c1 = (tl - tp) * (tl - tm1) / ((t0 - tp) * (t0 - tm1));
   T.c2 = (tl - tp) * (tl - t0) / ((tm1 - tp) * (tm1 - t0));
-  const bool time0 = (L.t == 0.0);
-  double* yf = L.slab[L.in_step ? L.last_out : L.cur];
   const int tab = T.same ? 0 : 1;
-  RC(launch_finebound(L.G, Lc.G, h->phys, T, h->shift, h->scalar, time0, L.fbtab[tab], L.fbn[tab], Lc.slab[Lc.cur],
-                      yf, strip_ptr(h, L, 0, 0), strip_ptr(h, L, 1, 0), strip_ptr(h, L, 0, 1),
-                      strip_ptr(h, L, 1, 1), strip_ptr(h, L, 0, 2), strip_ptr(h, L, 1, 2), h->stream));
-  return OA_OK;
+  it->Gf = L.G;
+  it->Gc = Lc.G;
+  it->P = h->phys;
+  it->time0 = (L.t == 0.0) ? 1 : 0;
+  it->table = L.fbtab[tab];
+  it->n = L.fbn[tab];
+  it->yc = Lc.slab[Lc.cur];
+  it->yf = L.slab[L.in_step ? L.last_out : L.cur];
+  it->rR0 = strip_ptr(h, L, 0, 0);
+  it->rR1 = strip_ptr(h, L, 1, 0);
+  it->zL0 = strip_ptr(h, L, 0, 1);
+  it->zL1 = strip_ptr(h, L, 1, 1);
+  it->zR0 = strip_ptr(h, L, 0, 2);
+  it->zR1 = strip_ptr(h, L, 1, 2);
+  return true;
 }
 
-int do_auxiliary(oa_handle* h, int level) {
+void prepare_auxiliary(oa_handle* h, int level, ProjItem* pj, bool* proj, DelItem* dd, bool* del) {
   Level& L = h->L[level];
   const bool time0 = (L.t == 0.0);
   double* y = L.slab[L.in_step ? L.last_out : L.cur];
   L.dtb_pre_valid = false;   // derivatives are re-formed from the current state
+  *proj = L.dirty;
   if (L.dirty) {
-    RC(launch_project(L.G, h->phys, h->shift, h->scalar, time0, y, h->stream));
+    *pj = ProjItem{};
+    pj->G = L.G;
+    pj->P = h->phys;
+    pj->time0 = time0 ? 1 : 0;
+    pj->y = y;
     L.dirty = false;
     L.aux_phi = time0;
     if (!time0 && h->p.chimethod) L.phi_valid = false;
   }
-  if (time0) RC(launch_deldef(L.G, y, h->stream));
-  return OA_OK;
+  *del = time0;
+  if (time0) {
+    *dd = DelItem{};
+    dd->G = L.G;
+    dd->y = y;
+  }
 }
 
-int do_restrict(oa_handle* h, int level) {
+void prepare_restrict(oa_handle* h, int level, RsItem* it) {
   Level& L = h->L[level];
   Level& Lc = h->L[level - 1];
-  RC(launch_restrict(L.G, Lc.G, h->nf, h->p.interporder, L.slab[L.cur], Lc.slab[Lc.cur], h->stream));
+  *it = RsItem{};
+  it->Gf = L.G;
+  it->Gc = Lc.G;
+  it->nf = h->nf;
+  it->interporder = h->p.interporder;
+  it->yf = L.slab[L.cur];
+  it->yc = Lc.slab[Lc.cur];
   Lc.dirty = true;
-  return OA_OK;
 }
 
-void advance_time(Level& L) {
+void finish_level_step(oa_handle* h, int level) {
+  Level& L = h->L[level];
   L.t2 = L.t1;
   L.t1 = L.t;
   L.s = L.s + 1;
   L.t = L.t + L.G.dt;
+  h->point_updates += static_cast<double>(h->p.Nrtotal) * static_cast<double>(h->p.Nztotal);
 }
 
-// onestep(level): onestep.f90:78-469 for box 0
-int onestep(oa_handle* h, int level) {
-  Level& L = h->L[level];
-  RC(do_saveold(h, level));
-  const bool rk4 = (h->p.integrator == OA_RK4);
-  const int niter = rk4 ? 4 : h->p.icniter;
-  const bool time0 = (L.t == 0.0);
-  for (int iter = 1; iter <= niter; ++iter) {
-    double dtw, w = 0.0;
-    if (rk4) {
-      switch (iter) {
-        case 1: dtw = 0.5 * L.G.dt; w = 1.0 / 6.0; break;
-        case 2: dtw = 0.5 * L.G.dt; w = 1.0 / 3.0; break;
-        case 3: dtw = L.G.dt; w = 1.0 / 3.0; break;
-        default: dtw = L.G.dt; w = 1.0 / 6.0; break;
-      }
-    } else {
-      dtw = (iter < niter) ? 0.5 * L.G.dt : L.G.dt;
-    }
-    RC(do_substep(h, level, iter, niter, dtw, w));
-    if (level > 0) RC(do_finebound(h, level, dtw));
-    if (time0) RC(launch_deldef(L.G, L.slab[L.last_out], h->stream));
-  }
-  advance_time(L);
-  h->point_updates += static_cast<double>(h->p.Nrtotal) * static_cast<double>(h->p.Nztotal);
-  if (level < h->nlev - 1) {
-    RC(onestep(h, level + 1));
-    RC(onestep(h, level + 1));
-  }
-  if (level > 0 && (L.s % 2) == 0) {
-    RC(do_restrict(h, level));
-    RC(do_auxiliary(h, level - 1));
-  }
-  return OA_OK;
+}  // namespace oax
+
+namespace {
+
+// The piecewise entry points: one launch each, through the handle's plan.
+int do_saveold(oa_handle* h, int level) {
+  SaveItem it;
+  RC(prepare_saveold(h, level, &it));
+  h->plan.begin();
+  h->plan.add_one(K_SAVE, false, false, false, it);
+  return h->plan.submit(h->stream);
+}
+
+int do_substep(oa_handle* h, int level, int iter, int niter, double dtw, double w) {
+  RhsItem it;
+  bool phimode = false;
+  RC(prepare_substep(h, level, iter, niter, dtw, w, &it, &phimode));
+  h->plan.begin();
+  h->plan.add_one(K_RHS, h->shift, h->scalar, phimode, it);
+  return h->plan.submit(h->stream);
+}
+
+int do_finebound(oa_handle* h, int level, double dtw) {
+  FbItem it;
+  if (!prepare_finebound(h, level, dtw, &it)) return OA_OK;
+  h->plan.begin();
+  h->plan.add_one(K_FB, h->shift, h->scalar, false, it);
+  return h->plan.submit(h->stream);
+}
+
+int do_auxiliary(oa_handle* h, int level) {
+  ProjItem pj;
+  DelItem dd;
+  bool proj = false, del = false;
+  prepare_auxiliary(h, level, &pj, &proj, &dd, &del);
+  h->plan.begin();
+  if (proj) h->plan.add_one(K_PROJ, h->shift, h->scalar, false, pj);
+  if (del) h->plan.add_one(K_DEL, false, false, false, dd);
+  return h->plan.submit(h->stream);
+}
+
+int do_restrict(oa_handle* h, int level) {
+  RsItem it;
+  prepare_restrict(h, level, &it);
+  h->plan.begin();
+  h->plan.add_one(K_RS, false, false, false, it);
+  return h->plan.submit(h->stream);
+}
+
+// one coarse step: onestep(0) (onestep.f90:78-469) planned as a whole and
+// submitted with one transfer of its work items
+int coarse_step(oa_handle* h) {
+  h->plan.begin();
+  RC(plan_level(h->plan, std::vector<oa_handle*>{h}, 0));
+  return h->plan.submit(h->stream);
 }
 
 void fill_grid(KGrid& G, const oa_params& p, int level) {
@@ -352,10 +375,16 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
     oa_set_error("no HIP device available: this library runs on the GPU only");
     return OA_ERR_HIP;
   }
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev_begin) != hipSuccess || hipEventCreate(&h->ev_end) != hipSuccess) {
     delete h;
     oa_set_error("HIP stream/event creation failed");
+    return OA_ERR_HIP;
+  }
+  h->stream = h->own_stream;
+  if (h->plan.init() != OA_OK) {
+    oa_destroy(h);
+    oa_set_error("HIP event creation failed");
     return OA_ERR_HIP;
   }
   h->L.resize(h->nlev);
@@ -404,6 +433,7 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
 
 extern "C" int oa_destroy(oa_handle* h) {
   if (!h) return OA_OK;
+  if (h->batch) (void)oa_batch_remove(h->batch, h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (Level& L : h->L) {
     for (int b = 0; b < 3; ++b) {
@@ -416,10 +446,10 @@ extern "C" int oa_destroy(oa_handle* h) {
     for (int k = 0; k < 2; ++k)
       if (L.fbtab[k]) (void)hipFree(L.fbtab[k]);
   }
-  for (hipEvent_t e : h->kev) (void)hipEventDestroy(e);
+  h->plan.release();
   if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);
   if (h->ev_end) (void)hipEventDestroy(h->ev_end);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
   return OA_OK;
 }
@@ -586,7 +616,10 @@ extern "C" int oa_restrict(oa_handle* h, int box, int level) {
 extern "C" int oa_advance_time(oa_handle* h, int box, int level) {
   Level* L = level_of(h, box, level);
   if (!L) return OA_ERR_ARG;
-  advance_time(*L);
+  L->t2 = L->t1;
+  L->t1 = L->t;
+  L->s = L->s + 1;
+  L->t = L->t + L->G.dt;
   return OA_OK;
 }
 
@@ -611,10 +644,17 @@ extern "C" int oa_initial_aux(oa_handle* h) {
   return OA_OK;
 }
 
+int refuse_member(const oa_handle* h, const char* what) {
+  oa_set_error(std::string(what) + ": the spacetime is a member of a batch; step it with oa_batch_coarse_step");
+  (void)h;
+  return OA_ERR_STATE;
+}
+
 extern "C" int oa_coarse_step(oa_handle* h) {
   if (!h) return OA_ERR_ARG;
+  if (h->batch) return refuse_member(h, "oa_coarse_step");
   HIPCHK(hipEventRecord(h->ev_begin, h->stream));
-  RC(onestep(h, 0));
+  RC(coarse_step(h));
   HIPCHK(hipEventRecord(h->ev_end, h->stream));
   HIPCHK(hipEventSynchronize(h->ev_end));
   float ms = 0.0f;
@@ -625,27 +665,23 @@ extern "C" int oa_coarse_step(oa_handle* h) {
 
 extern "C" int oa_evolve(oa_handle* h, int nsteps) {
   if (!h || nsteps < 0) return OA_ERR_ARG;
+  if (h->batch) return refuse_member(h, "oa_evolve");
   for (Level& L : h->L)
     if (L.dirty) {
       oa_set_error("oa_evolve: state not projected; call oa_initial_aux after uploading");
       return OA_ERR_STATE;
     }
-  h->kev_used = 0;
+  h->plan.reset_timing();
   HIPCHK(hipEventRecord(h->ev_begin, h->stream));
-  for (int s = 0; s < nsteps; ++s) RC(onestep(h, 0));
+  for (int s = 0; s < nsteps; ++s) RC(coarse_step(h));
   HIPCHK(hipEventRecord(h->ev_end, h->stream));
   HIPCHK(hipEventSynchronize(h->ev_end));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, h->ev_begin, h->ev_end));
   h->last_ms = ms;
-  if (h->kernel_timing && h->kev_used > 0) {
+  if (h->plan.kernel_timing && h->plan.kev_used > 0) {
     double sum = 0.0;
-    for (size_t q = 0; q + 1 < h->kev_used; q += 2) {
-      float m = 0.0f;
-      HIPCHK(hipEventElapsedTime(&m, h->kev[q], h->kev[q + 1]));
-      sum += m;
-    }
-    h->rhs_launches = static_cast<long>(h->kev_used / 2);
+    RC(h->plan.rhs_stats(&sum, &h->rhs_launches, nullptr));
     h->rhs_avg_ms = sum / static_cast<double>(h->rhs_launches);
   }
   return OA_OK;
@@ -665,7 +701,7 @@ extern "C" int oa_last_elapsed_ms(const oa_handle* h, double* ms) {
 
 extern "C" int oa_set_kernel_timing(oa_handle* h, int on) {
   if (!h) return OA_ERR_ARG;
-  h->kernel_timing = (on != 0);
+  h->plan.kernel_timing = (on != 0);
   return OA_OK;
 }
 

@@ -26,6 +26,12 @@
 
 namespace oax {
 
+#define RC_K(call)                \
+  do {                            \
+    const int rc_ = (call);       \
+    if (rc_ != OA_OK) return rc_; \
+  } while (0)
+
 // ------------------------------------------------------------ RHS kernel
 // Each 16 x 16 block stages its tile of every field, with a 3-point halo, in
 // LDS and reads all stencils from there.  Centred stencils reach 3 points.
@@ -66,6 +72,19 @@ __device__ __forceinline__ void stage_tile(const KGrid& G, double* tile, int nst
   __syncthreads();
 }
 
+// the item of block b in a table launch: the last one whose block0 <= b
+// (block0 ascends; the loads are uniform across the block)
+template <class Item>
+__device__ __forceinline__ const Item& find_item(const Item* __restrict__ items, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return items[lo];
+}
+
 // One RK substep of a level: each 16 x 16 block stages its tile and runs two
 // phases per point.  Phase 1 forms the geometric auxiliaries (D_i D_j alpha,
 // the Ricci tensor, div KT: the longest expressions, which need only the
@@ -74,18 +93,17 @@ __device__ __forceinline__ void stage_tile(const KGrid& G, double* tile, int nst
 // update.  Kept apart, the phases need at most the registers of the larger
 // one, so the kernel runs at two waves per SIMD without spilling.
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
-__global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KStep S, const double* __restrict__ yin,
-                                                          const double* __restrict__ yp, double* __restrict__ acc,
-                                                          double* __restrict__ yout, double* __restrict__ gaux) {
-  extern __shared__ double tile[];
+__device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, double* tile) {
   constexpr int nf = nslots(SHIFT, SCALAR);
-  const int nbj = static_cast<int>(gridDim.x), nbi = static_cast<int>(gridDim.y);
-  const int bj = static_cast<int>(blockIdx.x), bi = static_cast<int>(blockIdx.y);
+  const KGrid& G = it.G;
+  const KStep& S = it.S;
+  const int nbj = it.nbj, nbi = it.nbi;
   const int j0 = tile_start(bj, nbj, G.jlo, G.Nz);
   const int i0 = tile_start(bi, nbi, 1, G.Nr);
   const int ib = i0 - kHalo, jb = j0 - kHalo;
   // the evolved fields, then (first substep only) the dtbeta planes it differentiates
   const int nst = nf + (S.dtb_pre ? 2 : 0);
+  const double* __restrict__ yin = it.yin;
   stage_tile(G, tile, nst, [&](int f) { return (f < nf) ? yin + f * G.plane : S.dtb_pre + (f - nf) * G.plane; },
              ib, jb);
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
@@ -97,27 +115,108 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs(KGrid G, KPhys P, KSte
   // carrying phase-1 values or loads into phase 2, so the two phases do not
   // add up in registers
   if (interior_tile(G, i0, j0))
-    rhs_point<false, false, PHIMODE, true, 1>(G, P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+    rhs_point<false, false, PHIMODE, true, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
   else
-    rhs_point<false, false, PHIMODE, false, 1>(G, P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+    rhs_point<false, false, PHIMODE, false, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
   asm volatile("" ::: "memory");
-  rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, P, S, V, yp, acc, yout, gaux, i, j);
+  rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, it.P, S, V, it.yp, it.acc, it.yout, it.gaux, i, j);
 }
 
-// ------------------------------------------------------- projection pass
+template <bool SHIFT, bool SCALAR, bool PHIMODE>
+__global__ void __launch_bounds__(kTile * kTile, 2) k_rhs_table(const RhsItem* __restrict__ items, int n) {
+  extern __shared__ double tile[];
+  const int b = static_cast<int>(blockIdx.x);
+  const RhsItem& it = find_item(items, n, b);
+  const int t = b - it.block0;
+  rhs_tile<SHIFT, SCALAR, PHIMODE>(it, t / it.nbj, t % it.nbj, tile);
+}
+
+// ----------------------------------------------- pointwise kernels
+// One thread per point (or record) of an item, 256 threads per block; q is
+// the thread's index within its item.
+constexpr int kPtBlock = 256;
+
+__device__ __forceinline__ bool level_point(const KGrid& G, long q, int& i, int& j) {
+  const int nj = G.Nz - G.jlo + 1;
+  i = 1 + static_cast<int>(q / nj);
+  j = G.jlo + static_cast<int>(q % nj);
+  return i <= G.Nr;
+}
+
 template <bool SHIFT, bool SCALAR>
-__global__ void __launch_bounds__(256) k_project(KGrid G, KPhys P, int time0, double* __restrict__ y) {
-  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-  if (j > G.Nz || i > G.Nr) return;
-  project_point<SHIFT, SCALAR>(G, P, time0, y, i, j);
+__device__ __forceinline__ void project_body(const ProjItem& it, long q) {
+  int i, j;
+  if (level_point(it.G, q, i, j)) project_point<SHIFT, SCALAR>(it.G, it.P, it.time0, it.y, i, j);
 }
 
-__global__ void __launch_bounds__(256) k_deldef(KGrid G, double* __restrict__ y) {
-  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-  if (j > G.Nz || i > G.Nr) return;
-  deldef_point(G, y, i, j);
+__device__ __forceinline__ void deldef_body(const DelItem& it, long q) {
+  int i, j;
+  if (level_point(it.G, q, i, j)) deldef_point(it.G, it.y, i, j);
+}
+
+// saveold for the fine-boundary strips: blocks of one field, then the next
+__device__ __forceinline__ int save_per_field(const KGrid& G) {
+  const int nrt = G.Nr + G.g, nzt = G.Nz + G.g;
+  return G.g * (nrt > nzt ? nrt : nzt);
+}
+__device__ __forceinline__ void saveold_body(const SaveItem& it, int blk) {
+  const int per = save_per_field(it.G);
+  const int bpf = (per + kPtBlock - 1) / kPtBlock;
+  const int f = blk / bpf;
+  const int q = (blk % bpf) * kPtBlock + static_cast<int>(threadIdx.x);
+  if (f < it.nf && q < per) saveold_point(it.G, it.y, it.rR, it.zL, it.zR, f, q);
+}
+
+// finebound: blocks of one field group, then the next (the group stays
+// uniform within a block)
+template <bool SHIFT, bool SCALAR>
+__device__ __forceinline__ void finebound_body(const FbItem& it, int blk) {
+  const int bpg = static_cast<int>((it.n + kPtBlock - 1) / kPtBlock);
+  const int grp = blk / bpg;
+  const long q = static_cast<long>(blk % bpg) * kPtBlock + threadIdx.x;
+  if (grp >= finebound_groups(SHIFT, SCALAR) || q >= it.n) return;
+  finebound_point<SHIFT, SCALAR>(it.Gf, it.Gc, it.P, it.T, it.time0, it.yc, it.yf, it.rR0, it.rR1, it.zL0, it.zL1,
+                                 it.zR0, it.zR1, it.table[q], grp);
+}
+
+// restrict: blocks of one field, then the next; one thread per coarse point
+__device__ __forceinline__ void restrict_body(const RsItem& it, int blk) {
+  const KGrid& Gc = it.Gc;
+  const int nzt = Gc.Nz + Gc.g;
+  const long per = static_cast<long>(Gc.Nr + Gc.g) * nzt;
+  const int bpf = static_cast<int>((per + kPtBlock - 1) / kPtBlock);
+  const int s = blk / bpf;
+  const long q = static_cast<long>(blk % bpf) * kPtBlock + threadIdx.x;
+  if (s >= it.nf || q >= per) return;
+  const int ic = static_cast<int>(q / nzt) + 1 - Gc.g, jc = static_cast<int>(q % nzt) + 1 - Gc.g;
+  restrict_point(it.Gf, Gc, s, it.interporder, it.yf, it.yc, ic, jc);
+}
+
+template <bool SHIFT, bool SCALAR>
+__global__ void __launch_bounds__(kPtBlock) k_project_table(const ProjItem* __restrict__ items, int n) {
+  const ProjItem& it = find_item(items, n, static_cast<int>(blockIdx.x));
+  project_body<SHIFT, SCALAR>(it, static_cast<long>(blockIdx.x - it.block0) * kPtBlock + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(kPtBlock) k_deldef_table(const DelItem* __restrict__ items, int n) {
+  const DelItem& it = find_item(items, n, static_cast<int>(blockIdx.x));
+  deldef_body(it, static_cast<long>(blockIdx.x - it.block0) * kPtBlock + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(kPtBlock) k_saveold_table(const SaveItem* __restrict__ items, int n) {
+  const SaveItem& it = find_item(items, n, static_cast<int>(blockIdx.x));
+  saveold_body(it, static_cast<int>(blockIdx.x) - it.block0);
+}
+
+template <bool SHIFT, bool SCALAR>
+__global__ void __launch_bounds__(kPtBlock) k_finebound_table(const FbItem* __restrict__ items, int n) {
+  const FbItem& it = find_item(items, n, static_cast<int>(blockIdx.x));
+  finebound_body<SHIFT, SCALAR>(it, static_cast<int>(blockIdx.x) - it.block0);
+}
+
+__global__ void __launch_bounds__(kPtBlock) k_restrict_table(const RsItem* __restrict__ items, int n) {
+  const RsItem& it = find_item(items, n, static_cast<int>(blockIdx.x));
+  restrict_body(it, static_cast<int>(blockIdx.x) - it.block0);
 }
 
 // dtbeta = drivercsi * Delta (evolve.f90:174-195), whole arrays incl. images
@@ -127,29 +226,6 @@ __global__ void k_dtbeta_init(long n, double csi, const double* __restrict__ dr,
   if (q >= n) return;
   tr[q] = csi * dr[q];
   tz[q] = csi * dz[q];
-}
-
-// saveold for the fine-boundary strips (one thread per strip element)
-__global__ void k_saveold(KGrid G, int nf, const double* __restrict__ y, double* __restrict__ rR,
-                          double* __restrict__ zL, double* __restrict__ zR) {
-  const int q = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  const int f = blockIdx.y;
-  if (f >= nf) return;
-  saveold_point(G, y, rR, zL, zR, f, q);
-}
-
-// finebound(box 0, level, dtw, .true.): one thread per fine strip point
-template <bool SHIFT, bool SCALAR>
-__global__ void __launch_bounds__(256) k_finebound(KGrid Gf, KGrid Gc, KPhys P, KTime T, int time0,
-                                                   const FbPoint* __restrict__ table, long n,
-                                                   const double* __restrict__ yc, double* __restrict__ yf,
-                                                   const double* __restrict__ rR0, const double* __restrict__ rR1,
-                                                   const double* __restrict__ zL0, const double* __restrict__ zL1,
-                                                   const double* __restrict__ zR0, const double* __restrict__ zR1) {
-  const long q = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  finebound_point<SHIFT, SCALAR>(Gf, Gc, P, T, time0, yc, yf, rR0, rR1, zL0, zL1, zR0, zR1, table[q],
-                                 static_cast<int>(blockIdx.y));
 }
 
 // the fixed geometry of the fine boundary zone of depth D (once per level)
@@ -162,21 +238,7 @@ __global__ void __launch_bounds__(256) k_fb_table(KGrid Gf, KGrid Gc, int D, int
   table[q] = fp;
 }
 
-// restrict(box 0, level, .true.): one thread per (coarse point, field)
-__global__ void __launch_bounds__(256) k_restrict(KGrid Gf, KGrid Gc, int interporder,
-                                                  const double* __restrict__ yf, double* __restrict__ yc) {
-  const int jc = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + 1 - Gc.g;
-  const int ic = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y) + 1 - Gc.g;
-  if (jc > Gc.Nz || ic > Gc.Nr) return;
-  restrict_point(Gf, Gc, static_cast<int>(blockIdx.z), interporder, yf, yc, ic, jc);
-}
-
 // ------------------------------------------------------------- launchers
-static dim3 level_grid(const KGrid& G, dim3 blk) {
-  const int nj = G.Nz - G.jlo + 1, ni = G.Nr;
-  return dim3((nj + blk.x - 1) / blk.x, (ni + blk.y - 1) / blk.y, 1);
-}
-
 static int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -184,6 +246,33 @@ static int check_launch() {
     return OA_ERR_HIP;
   }
   return OA_OK;
+}
+
+static int blocks_for(long threads) { return static_cast<int>((threads + kPtBlock - 1) / kPtBlock); }
+
+int rhs_tiles(const KGrid& G, int* nbi, int* nbj) {
+  const int nj = G.Nz - G.jlo + 1;
+  if (G.Nr < kTile || nj < kTile) {
+    oa_set_error("RHS kernel: a level needs at least 16 points in each direction");
+    return OA_ERR_ARG;
+  }
+  *nbi = (G.Nr + kTile - 1) / kTile;
+  *nbj = (nj + kTile - 1) / kTile;
+  return OA_OK;
+}
+
+int rhs_blocks(const RhsItem& it) { return it.nbi * it.nbj; }
+int proj_blocks(const ProjItem& it) { return blocks_for(static_cast<long>(it.G.Nr) * (it.G.Nz - it.G.jlo + 1)); }
+int del_blocks(const DelItem& it) { return blocks_for(static_cast<long>(it.G.Nr) * (it.G.Nz - it.G.jlo + 1)); }
+int save_blocks(const SaveItem& it) {
+  const int nrt = it.G.Nr + it.G.g, nzt = it.G.Nz + it.G.g;
+  return it.nf * blocks_for(static_cast<long>(it.G.g) * (nrt > nzt ? nrt : nzt));
+}
+int fb_blocks(const FbItem& it, bool shift, bool scalar) {
+  return finebound_groups(shift, scalar) * blocks_for(it.n);
+}
+int rs_blocks(const RsItem& it) {
+  return it.nf * blocks_for(static_cast<long>(it.Gc.Nr + it.Gc.g) * (it.Gc.Nz + it.Gc.g));
 }
 
 template <class K>
@@ -196,81 +285,89 @@ static int raise_lds_limit(K* kernel, size_t bytes) {
   return OA_OK;
 }
 
+// the dynamic LDS of one RHS block exceeds the 64 KiB default; two more
+// planes on a level's first substep (KStep::dtb_pre)
 template <bool SH, bool SC, bool PM>
-static int launch_rhs_t(const KGrid& G, const KPhys& P, const KStep& S, const double* yin, const double* yp,
-                        double* acc, double* yout, double* gaux, hipStream_t st) {
-  // the dynamic LDS of one block exceeds the 64 KiB default; two more
-  // planes on a level's first substep (KStep::dtb_pre)
+static int rhs_lds(bool dtb_pre, size_t* lds) {
   const size_t per = sizeof(double) * kTileSize;
-  const size_t lds = per * static_cast<size_t>(nslots(SH, SC) + (S.dtb_pre ? 2 : 0));
   static bool attr_set = false;
   if (!attr_set) {
-    const int rc = raise_lds_limit(&k_rhs<SH, SC, PM>, per * static_cast<size_t>(nslots(SH, SC) + 2));
-    if (rc != OA_OK) return rc;
+    RC_K(raise_lds_limit(&k_rhs_table<SH, SC, PM>, per * static_cast<size_t>(nslots(SH, SC) + 2)));
     attr_set = true;
   }
-  const dim3 blk(kTile, kTile, 1);
-  const int nj = G.Nz - G.jlo + 1;
-  const dim3 grd(static_cast<unsigned>((nj + kTile - 1) / kTile), static_cast<unsigned>((G.Nr + kTile - 1) / kTile), 1);
-  hipLaunchKernelGGL((k_rhs<SH, SC, PM>), grd, blk, lds, st, G, P, S, yin, yp, acc, yout, gaux);
+  *lds = per * static_cast<size_t>(nslots(SH, SC) + (dtb_pre ? 2 : 0));
+  return OA_OK;
+}
+
+template <bool SH, bool SC, bool PM>
+static int launch_rhs_table_t(const RhsItem* items, int n, int blocks, bool any_pre, hipStream_t st) {
+  size_t lds = 0;
+  RC_K((rhs_lds<SH, SC, PM>(any_pre, &lds)));
+  hipLaunchKernelGGL((k_rhs_table<SH, SC, PM>), dim3(blocks), dim3(kTile, kTile, 1), lds, st, items, n);
   return check_launch();
 }
 
-int launch_rhs(const KGrid& G, const KPhys& P, const KStep& S, bool shift, bool scalar, bool phimode,
-               const double* yin, const double* yp, double* acc, double* yout, double* gaux, void* stream) {
-  if (G.Nr < kTile || G.Nz - G.jlo + 1 < kTile) {
-    oa_set_error("launch_rhs: a level needs at least 16 points in each direction");
-    return OA_ERR_ARG;
+#define RHS_DISPATCH(FN, ...)                                   \
+  switch ((shift ? 4 : 0) + (scalar ? 2 : 0) + (phimode ? 1 : 0)) { \
+    case 0: return FN<false, false, false>(__VA_ARGS__);        \
+    case 1: return FN<false, false, true>(__VA_ARGS__);         \
+    case 2: return FN<false, true, false>(__VA_ARGS__);         \
+    case 3: return FN<false, true, true>(__VA_ARGS__);          \
+    case 4: return FN<true, false, false>(__VA_ARGS__);         \
+    case 5: return FN<true, false, true>(__VA_ARGS__);          \
+    case 6: return FN<true, true, false>(__VA_ARGS__);          \
+    default: return FN<true, true, true>(__VA_ARGS__);          \
   }
+
+int launch_rhs_table(const RhsItem* items, int n, int blocks, bool shift, bool scalar, bool phimode,
+                     bool any_dtb_pre, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int code = (shift ? 4 : 0) + (scalar ? 2 : 0) + (phimode ? 1 : 0);
-  switch (code) {
-    case 0: return launch_rhs_t<false, false, false>(G, P, S, yin, yp, acc, yout, gaux, st);
-    case 1: return launch_rhs_t<false, false, true>(G, P, S, yin, yp, acc, yout, gaux, st);
-    case 2: return launch_rhs_t<false, true, false>(G, P, S, yin, yp, acc, yout, gaux, st);
-    case 3: return launch_rhs_t<false, true, true>(G, P, S, yin, yp, acc, yout, gaux, st);
-    case 4: return launch_rhs_t<true, false, false>(G, P, S, yin, yp, acc, yout, gaux, st);
-    case 5: return launch_rhs_t<true, false, true>(G, P, S, yin, yp, acc, yout, gaux, st);
-    case 6: return launch_rhs_t<true, true, false>(G, P, S, yin, yp, acc, yout, gaux, st);
-    default: return launch_rhs_t<true, true, true>(G, P, S, yin, yp, acc, yout, gaux, st);
-  }
+  RHS_DISPATCH(launch_rhs_table_t, items, n, blocks, any_dtb_pre, st)
 }
 
-int launch_project(const KGrid& G, const KPhys& P, bool shift, bool scalar, bool time0, double* y, void* stream) {
-  const dim3 blk(64, 4, 1);
-  const dim3 grd = level_grid(G, blk);
+#define SC_DISPATCH(K, GRID, ...)                                                                  \
+  do {                                                                                             \
+    if (shift && scalar) hipLaunchKernelGGL((K<true, true>), GRID, dim3(kPtBlock), 0, st, __VA_ARGS__);   \
+    else if (shift) hipLaunchKernelGGL((K<true, false>), GRID, dim3(kPtBlock), 0, st, __VA_ARGS__);       \
+    else if (scalar) hipLaunchKernelGGL((K<false, true>), GRID, dim3(kPtBlock), 0, st, __VA_ARGS__);      \
+    else hipLaunchKernelGGL((K<false, false>), GRID, dim3(kPtBlock), 0, st, __VA_ARGS__);                 \
+  } while (0)
+
+int launch_project_table(const ProjItem* items, int n, int blocks, bool shift, bool scalar, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int t0 = time0 ? 1 : 0;
-  if (shift && scalar) hipLaunchKernelGGL((k_project<true, true>), grd, blk, 0, st, G, P, t0, y);
-  else if (shift) hipLaunchKernelGGL((k_project<true, false>), grd, blk, 0, st, G, P, t0, y);
-  else if (scalar) hipLaunchKernelGGL((k_project<false, true>), grd, blk, 0, st, G, P, t0, y);
-  else hipLaunchKernelGGL((k_project<false, false>), grd, blk, 0, st, G, P, t0, y);
+  SC_DISPATCH(k_project_table, dim3(blocks), items, n);
   return check_launch();
 }
 
-int launch_deldef(const KGrid& G, double* y, void* stream) {
-  const dim3 blk(64, 4, 1);
-  hipLaunchKernelGGL(k_deldef, level_grid(G, blk), blk, 0, static_cast<hipStream_t>(stream), G, y);
+int launch_finebound_table(const FbItem* items, int n, int blocks, bool shift, bool scalar, void* stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  SC_DISPATCH(k_finebound_table, dim3(blocks), items, n);
+  return check_launch();
+}
+
+int launch_deldef_table(const DelItem* items, int n, int blocks, void* stream) {
+  hipLaunchKernelGGL(k_deldef_table, dim3(blocks), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), items, n);
+  return check_launch();
+}
+
+int launch_saveold_table(const SaveItem* items, int n, int blocks, void* stream) {
+  hipLaunchKernelGGL(k_saveold_table, dim3(blocks), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), items, n);
+  return check_launch();
+}
+
+int launch_restrict_table(const RsItem* items, int n, int blocks, void* stream) {
+  hipLaunchKernelGGL(k_restrict_table, dim3(blocks), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), items,
+                     n);
   return check_launch();
 }
 
 int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream) {
   (void)scalar;
   const long n = G.plane;
-  const int blk = 256;
   const long pl = G.plane;
-  hipLaunchKernelGGL(k_dtbeta_init, dim3(static_cast<unsigned>((n + blk - 1) / blk)), dim3(blk), 0,
-                     static_cast<hipStream_t>(stream), n, csi, y + S_Delta_r * pl, y + S_Delta_z * pl,
-                     y + shift_slot(SH_dtbeta_r) * pl, y + shift_slot(SH_dtbeta_z) * pl);
-  return check_launch();
-}
-
-int launch_saveold(const KGrid& G, int nf, const double* y, double* rR, double* zL, double* zR, void* stream) {
-  const int nrt = G.Nr + G.g, nzt = G.Nz + G.g;
-  const int per = G.g * (nrt > nzt ? nrt : nzt);
-  const int blk = 256;
-  hipLaunchKernelGGL(k_saveold, dim3((per + blk - 1) / blk, nf), dim3(blk), 0, static_cast<hipStream_t>(stream),
-                     G, nf, y, rR, zL, zR);
+  hipLaunchKernelGGL(k_dtbeta_init, dim3(blocks_for(n)), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), n,
+                     csi, y + S_Delta_r * pl, y + S_Delta_z * pl, y + shift_slot(SH_dtbeta_r) * pl,
+                     y + shift_slot(SH_dtbeta_z) * pl);
   return check_launch();
 }
 
@@ -279,36 +376,8 @@ size_t fb_record_bytes() { return sizeof(FbPoint); }
 
 int launch_fb_table(const KGrid& Gf, const KGrid& Gc, int depth, int interporder, FbPoint* table, long n,
                     void* stream) {
-  hipLaunchKernelGGL(k_fb_table, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), Gf, Gc, depth, interporder, n, table);
-  return check_launch();
-}
-
-int launch_finebound(const KGrid& Gf, const KGrid& Gc, const KPhys& P, const KTime& T, bool shift, bool scalar,
-                     bool time0, const FbPoint* table, long n, const double* yc, double* yf, const double* rR0,
-                     const double* rR1, const double* zL0, const double* zL1, const double* zR0, const double* zR1,
-                     void* stream) {
-  const int blk = 256;
-  const dim3 grd(static_cast<unsigned>((n + blk - 1) / blk), static_cast<unsigned>(finebound_groups(shift, scalar)), 1);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const int t0 = time0 ? 1 : 0;
-#define LAUNCH_FB(SH, SC) \
-  hipLaunchKernelGGL((k_finebound<SH, SC>), grd, dim3(blk), 0, st, Gf, Gc, P, T, t0, table, n, yc, yf, rR0, rR1, \
-                     zL0, zL1, zR0, zR1)
-  if (shift && scalar) LAUNCH_FB(true, true);
-  else if (shift) LAUNCH_FB(true, false);
-  else if (scalar) LAUNCH_FB(false, true);
-  else LAUNCH_FB(false, false);
-#undef LAUNCH_FB
-  return check_launch();
-}
-
-int launch_restrict(const KGrid& Gf, const KGrid& Gc, int nf, int interporder, const double* yf, double* yc,
-                    void* stream) {
-  const dim3 blk(64, 4, 1);
-  const dim3 grd((Gc.Nz + Gc.g + blk.x - 1) / blk.x, (Gc.Nr + Gc.g + blk.y - 1) / blk.y,
-                 static_cast<unsigned>(nf));
-  hipLaunchKernelGGL(k_restrict, grd, blk, 0, static_cast<hipStream_t>(stream), Gf, Gc, interporder, yf, yc);
+  hipLaunchKernelGGL(k_fb_table, dim3(blocks_for(n)), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), Gf, Gc,
+                     depth, interporder, n, table);
   return check_launch();
 }
 

@@ -195,6 +195,85 @@ class Spacetime:
         return v.value
 
 
+class Batch:
+    """Many independent spacetimes advanced together: each coarse step is one
+    launch per kernel kind over the (spacetime, level) work items of every
+    member.  Members keep their own state and are read with their own
+    Spacetime.download; results equal those of each member evolved alone."""
+
+    def __init__(self, members=()):
+        h = ctypes.c_void_p()
+        _check(lib().oa_batch_create(ctypes.byref(h)), "oa_batch_create")
+        self._b = h
+        self.members = []
+        for st in members:
+            self.add(st)
+
+    def add(self, st):
+        _check(lib().oa_batch_add(self._b, st._h), "oa_batch_add")
+        self.members.append(st)
+
+    def remove(self, st):
+        _check(lib().oa_batch_remove(self._b, st._h), "oa_batch_remove")
+        self.members.remove(st)
+
+    def __len__(self):
+        return len(self.members)
+
+    def coarse_step(self):
+        _check(lib().oa_batch_coarse_step(self._b), "oa_batch_coarse_step")
+
+    def evolve(self, nsteps):
+        _check(lib().oa_batch_evolve(self._b, int(nsteps)), "oa_batch_evolve")
+
+    def sync(self):
+        _check(lib().oa_batch_sync(self._b), "oa_batch_sync")
+
+    def last_elapsed_ms(self):
+        v = ctypes.c_double()
+        _check(lib().oa_batch_last_elapsed_ms(self._b, ctypes.byref(v)), "oa_batch_last_elapsed_ms")
+        return v.value
+
+    def point_updates(self):
+        v = ctypes.c_double()
+        _check(lib().oa_batch_point_updates(self._b, ctypes.byref(v)), "oa_batch_point_updates")
+        return v.value
+
+    def launch_stats(self):
+        n, m = ctypes.c_long(), ctypes.c_long()
+        _check(lib().oa_batch_launch_stats(self._b, ctypes.byref(n), ctypes.byref(m)), "oa_batch_launch_stats")
+        return n.value, m.value
+
+    def set_kernel_timing(self, on):
+        _check(lib().oa_batch_set_kernel_timing(self._b, 1 if on else 0), "oa_batch_set_kernel_timing")
+
+    def rhs_kernel_stats(self):
+        """(total ms, launches, grid points advanced) of the RHS launches of
+        the last timed evolve."""
+        ms, n, pts = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+        _check(lib().oa_batch_rhs_kernel_stats(self._b, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pts)),
+               "oa_batch_rhs_kernel_stats")
+        return ms.value, n.value, pts.value
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().oa_batch_destroy(self._b)
+            self._b = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 # names of the evolved state of a configuration (reference array names)
 BASE_FIELDS = ["alpha", "chi", "phi", "A", "B", "H", "C", "trK", "KTA", "KTB", "KTC", "Delta_r", "Delta_z"]
 SHIFT_FIELDS = ["beta_r", "beta_z", "dtbeta_r", "dtbeta_z"]

@@ -107,6 +107,19 @@ def load_library(path=LIBPATH):
         "oa_set_kernel_timing": (ctypes.c_int, [H, ctypes.c_int]),
         "oa_rhs_kernel_stats": (ctypes.c_int, [H, dp, ctypes.POINTER(ctypes.c_long)]),
         "oa_point_updates": (ctypes.c_int, [H, dp]),
+        "oa_batch_create": (ctypes.c_int, [ctypes.POINTER(H)]),
+        "oa_batch_destroy": (ctypes.c_int, [H]),
+        "oa_batch_add": (ctypes.c_int, [H, H]),
+        "oa_batch_remove": (ctypes.c_int, [H, H]),
+        "oa_batch_size": (ctypes.c_int, [H, ip]),
+        "oa_batch_coarse_step": (ctypes.c_int, [H]),
+        "oa_batch_evolve": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_batch_sync": (ctypes.c_int, [H]),
+        "oa_batch_last_elapsed_ms": (ctypes.c_int, [H, dp]),
+        "oa_batch_point_updates": (ctypes.c_int, [H, dp]),
+        "oa_batch_launch_stats": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
+        "oa_batch_set_kernel_timing": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_batch_rhs_kernel_stats": (ctypes.c_int, [H, dp, ctypes.POINTER(ctypes.c_long), dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
