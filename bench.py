@@ -66,23 +66,32 @@ def measured_traffic(workload):
     return None
 
 
-def cpu_baseline(text, steps):
-    """The oracle (plain C restatement, one core) on a bounded sample."""
+def cpu_baseline(texts, min_seconds):
+    """The oracle (plain C restatement, one core) on a bounded sample: coarse
+    steps of the given configurations in turn (one step each, cycling) until
+    min_seconds of CPU work have been timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oaxis
     import pyoracle
-    p = oaxis.params_from_text(text)
-    orc = pyoracle.Oracle(p)
-    orc.initial_data()
-    orc.initial_aux()
-    d = orc.dims(0)
-    per_coarse = sum(2 ** l for l in range(orc.nlevels)) * d["Nr"] * d["Nz"]
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    orcs = []
+    for text in texts:
+        orc = pyoracle.Oracle(oaxis.params_from_text(text))
+        orc.initial_data()
+        orc.initial_aux()
+        d = orc.dims(0)
+        orcs.append((orc, sum(2 ** l for l in range(orc.nlevels)) * d["Nr"] * d["Nz"]))
+    updates, dt, steps, k = 0.0, 0.0, 0, 0
+    while dt < min_seconds:
+        orc, per_coarse = orcs[k % len(orcs)]
+        t0 = time.perf_counter()
         orc.coarse_step()
-    dt = time.perf_counter() - t0
-    orc.close()
-    return per_coarse * steps / dt, dt
+        dt += time.perf_counter() - t0
+        updates += per_coarse
+        steps += 1
+        k += 1
+    for orc, _ in orcs:
+        orc.close()
+    return updates / dt, dt, steps
 
 
 def copy_probe(nbytes):
@@ -97,6 +106,22 @@ def copy_probe(nbytes):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": n * 8}))
+
+
+def run_serving(args, n, steps_scale, max_grid):
+    """The serving mix through the continuous-batching loop: Poisson
+    arrivals, admission by device footprint, retirement at the final step,
+    on lapse collapse or on dispersal.  Returns the loop's summary."""
+    from oaxis import serve
+    grids = tuple(g for g in ((64, 128), (128, 256), (256, 512), (512, 1024)) if g[0] <= max_grid)
+    lo, hi = max(1, int(200 * steps_scale)), max(1, int(2000 * steps_scale))
+    reqs = serve.serving_mix(args.seed, n, rate=args.serving_rate, grids=grids, steps=(lo, hi))
+    srv = serve.Server(memory_budget=args.serving_budget_gb * 1e9)
+    records, summary = srv.run(reqs)
+    summary["sample"] = ("%d requests of the serving mix (seed %d, Poisson rate %.3g/s, grids up to %dx%d, "
+                         "1-6 levels, %d-%d coarse steps), memory budget %.0f GB"
+                         % (n, args.seed, args.serving_rate, max_grid, 2 * max_grid, lo, hi, args.serving_budget_gb))
+    return summary
 
 
 def run_batched(args):
@@ -177,11 +202,17 @@ def run_batched(args):
                      "launches": rhs_n,
                      "algorithmic_bytes_per_launch": rhs_bytes / rhs_n if rhs_n else 0.0},
     }
+    if args.serving_requests > 0:
+        sv = run_serving(args, args.serving_requests, args.serving_steps_scale, 256)
+        rec["latency"] = {"p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "requests": sv["requests"],
+                          "updates_per_s": sv["updates_per_s"], "wall_s": sv["wall_s"],
+                          "retired": sv["reasons"], "sample": sv["sample"]}
     if not args.no_cpu_baseline:
-        v, secs = cpu_baseline(texts[0], args.cpu_steps)
+        sample = texts[:8]
+        v, secs, nst = cpu_baseline(sample, args.cpu_seconds)
         rec["cpu_baseline"] = {"value": v, "unit": "grid-point updates/s", "cores": 1, "kind": "port",
-                               "sample": "%d coarse step(s) of member 0 of the same workload on the oracle "
-                                         "(C restatement, 1 thread), %.1f s" % (args.cpu_steps, secs)}
+                               "sample": "%d coarse steps cycling over members 0-%d of the same workload on the "
+                                         "oracle (C restatement, 1 thread), %.1f s" % (nst, len(sample) - 1, secs)}
     print(json.dumps(rec))
 
 
@@ -191,11 +222,17 @@ def main():
     ap.add_argument("--steps", type=int, default=8, help="timed coarse steps")
     ap.add_argument("--warmup", type=int, default=2, help="untimed coarse steps")
     ap.add_argument("--workload", default="batched_scalar",
-                    help="batched_scalar (default), brill, plain_small, plain_256x512, plain_512x1024")
+                    help="batched_scalar (default), serving, brill, plain_small, plain_256x512, plain_512x1024")
     ap.add_argument("--members", type=int, default=64, help="spacetimes of the batched workload")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="coarse steps of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU time of the baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests of the serving sample reported as latency (0: none)")
+    ap.add_argument("--serving-steps-scale", type=float, default=0.1,
+                    help="scale of the 200-2000 coarse steps per request in the serving sample")
+    ap.add_argument("--serving-rate", type=float, default=20.0, help="Poisson arrival rate (1/s)")
+    ap.add_argument("--serving-budget-gb", type=float, default=200.0, help="device memory for admitted requests")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -211,6 +248,15 @@ def main():
 
     if args.workload == "batched_scalar":
         run_batched(args)
+        return
+    if args.workload == "serving":
+        # the full serving mix (grids up to 512x1024, 200-2000 steps scaled by --serving-steps-scale)
+        sv = run_serving(args, args.serving_requests, args.serving_steps_scale, 512)
+        print(json.dumps({"metric": METRIC, "value": sv["updates_per_s"], "unit": "grid-point updates/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic (seeded par generator, seed %d)" % args.seed,
+                          "config": {"workload": "serving mix", "sample": sv["sample"]},
+                          "latency": {k: sv[k] for k in ("p50_s", "p99_s", "requests", "wall_s", "reasons")}}))
         return
 
     import oaxis
@@ -269,10 +315,10 @@ def main():
     }
     st.close()
     if not args.no_cpu_baseline:
-        v, secs = cpu_baseline(text, args.cpu_steps)
+        v, secs, nst = cpu_baseline([text], args.cpu_seconds)
         rec["cpu_baseline"] = {"value": v, "unit": "grid-point updates/s", "cores": 1, "kind": "port",
                                "sample": "%d coarse step(s) of the same workload on the oracle (C restatement, "
-                                         "1 thread), %.1f s" % (args.cpu_steps, secs)}
+                                         "1 thread), %.1f s" % (nst, secs)}
     print(json.dumps(rec))
 
 

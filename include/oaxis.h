@@ -228,6 +228,17 @@ int oa_rhs_kernel_stats(const oa_handle* h, double* avg_ms, long* launches);
  * by this handle so far. */
 int oa_point_updates(const oa_handle* h, double* updates);
 
+/* 0D norms of an evolved array on one level, as save0Ddata.f90:221-288
+ * forms them over i in [1, Nr-ghost], j in [1, Nz-ghost]: out[0] = max,
+ * out[1] = min, out[2] = nm1 = max |f|, out[3] = nm2 = sqrt(sum f^2 / N).
+ * Reduced on the device (the sum in a fixed order that differs from the
+ * reference's serial loop, so nm2 agrees to rounding).  A NaN in the range
+ * returns OA_ERR_STATE (the reference stops the run there). */
+int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out);
+/* Device memory (bytes) oa_create allocates for these parameters: what a
+ * scheduler admitting spacetimes by HBM footprint needs. */
+int oa_params_footprint(const oa_params* p, double* bytes);
+
 /* ---- batches of spacetimes
  * No counterpart in the reference, which evolves one spacetime per run.
  * For callers that evolve many independent spacetimes at once (parameter

@@ -146,6 +146,9 @@ int launch_deldef_table(const DelItem* items, int n, int blocks, void* stream);
 int launch_saveold_table(const SaveItem* items, int n, int blocks, void* stream);
 int launch_finebound_table(const FbItem* items, int n, int blocks, bool shift, bool scalar, void* stream);
 int launch_restrict_table(const RsItem* items, int n, int blocks, void* stream);
+// 0D norms (save0Ddata): norms_blocks() partial records of 6 doubles
+int norms_blocks();
+int launch_norms_partial(const KGrid& G, const double* f, double* part, void* stream);
 int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream);
 int launch_fb_table(const KGrid& Gf, const KGrid& Gc, int depth, int interporder, FbPoint* table, long n,
                     void* stream);

@@ -49,6 +49,7 @@ struct oa_handle {
   oa_batch* batch = nullptr;
   oax::KPhys phys{};
   oax::Plan plan;                    // launches of this handle's own calls
+  double* norms_part = nullptr;      // partial records of the 0D norms
   // timing
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   double last_ms = 0.0;

@@ -387,6 +387,11 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
     oa_set_error("HIP event creation failed");
     return OA_ERR_HIP;
   }
+  if (hipMalloc(reinterpret_cast<void**>(&h->norms_part), sizeof(double) * 6 * norms_blocks()) != hipSuccess) {
+    oa_destroy(h);
+    oa_set_error("device allocation failed");
+    return OA_ERR_NOMEM;
+  }
   h->L.resize(h->nlev);
   for (int l = 0; l < h->nlev; ++l) {
     Level& L = h->L[l];
@@ -447,6 +452,7 @@ extern "C" int oa_destroy(oa_handle* h) {
       if (L.fbtab[k]) (void)hipFree(L.fbtab[k]);
   }
   h->plan.release();
+  if (h->norms_part) (void)hipFree(h->norms_part);
   if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);
   if (h->ev_end) (void)hipEventDestroy(h->ev_end);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
@@ -561,6 +567,66 @@ extern "C" int oa_download_field(oa_handle* h, int box, int level, const char* n
   }
   oa_set_error(std::string("oa_download_field: unknown array ") + name);
   return OA_ERR_ARG;
+}
+
+extern "C" int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out) {
+  Level* L = level_of(h, box, level);
+  if (!L || !name || !out) return OA_ERR_ARG;
+  const int slot = slot_of(h, name);
+  if (slot < 0 || (slot == S_phi && !L->phi_valid)) {
+    oa_set_error(std::string("oa_norms0d: ") + name + " is not a stored evolved array of this configuration");
+    return OA_ERR_ARG;
+  }
+  const double* y = L->slab[L->in_step ? L->last_out : L->cur];
+  RC(launch_norms_partial(L->G, y + static_cast<long>(slot) * L->G.plane, h->norms_part, h->stream));
+  const int nb = norms_blocks();
+  std::vector<double> part(static_cast<size_t>(6 * nb));
+  HIPCHK(hipMemcpyAsync(part.data(), h->norms_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double vmax = -1.0e300, vmin = 1.0e300, vabs = 0.0, sq = 0.0, cnt = 0.0, nan = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double* o = &part[static_cast<size_t>(6 * b)];
+    vmax = std::fmax(vmax, o[0]);
+    vmin = std::fmin(vmin, o[1]);
+    vabs = std::fmax(vabs, o[2]);
+    sq += o[3];
+    cnt += o[4];
+    nan = std::fmax(nan, o[5]);
+  }
+  if (nan != 0.0) {
+    oa_set_error(std::string("oa_norms0d: found NaN in ") + name);   // save0Ddata.f90:230-237 dies here
+    return OA_ERR_STATE;
+  }
+  out[0] = vmax;
+  out[1] = vmin;
+  out[2] = vabs;
+  out[3] = (cnt > 0.0) ? std::sqrt(sq / cnt) : 0.0;
+  return OA_OK;
+}
+
+extern "C" int oa_params_footprint(const oa_params* pin, double* bytes) {
+  if (!pin || !bytes) return OA_ERR_ARG;
+  oa_params p = *pin;
+  p.ghost = 3;
+  if (!p.eqsym && ((p.Nztotal + p.ghost) % 2 == 0)) p.Nztotal += 1;
+  const bool shift = (p.shift != OA_SHIFT_NONE), scalar = (p.mattertype & OA_MATTER_SCALAR) != 0;
+  const double nf = nslots(shift, scalar);
+  double total = sizeof(double) * 6.0 * norms_blocks();
+  for (int l = 0; l <= p.Nl0; ++l) {
+    KGrid G;
+    fill_grid(G, p, l);
+    const double plane = static_cast<double>(G.plane) * sizeof(double);
+    const double strips = static_cast<double>(G.g) * ((G.Nz + G.g) + 2.0 * (G.Nr + G.g)) * sizeof(double);
+    total += 4.0 * nf * plane;              // three state slabs and the RK4 accumulator
+    total += 3.0 * nf * strips;             // fine-boundary time levels
+    total += NAUX * plane;                  // geometric auxiliaries
+    if (shift) total += 2.0 * plane;        // pre-initialisation dtbeta
+    if (l > 0)
+      total += static_cast<double>(fb_record_bytes()) *
+               static_cast<double>(fb_points(G, G.g + 2) + fb_points(G, G.g - 1));
+  }
+  *bytes = total;
+  return OA_OK;
 }
 
 extern "C" int oa_get_time(const oa_handle* h, int box, int level, double* t, double* t1, double* t2, int* s) {

@@ -219,6 +219,48 @@ __global__ void __launch_bounds__(kPtBlock) k_restrict_table(const RsItem* __res
   restrict_body(it, static_cast<int>(blockIdx.x) - it.block0);
 }
 
+// 0D norms of one field over i in [1, Nr-g], j in [1, Nz-g] (the range
+// save0Ddata.f90:221-246 reduces): per block max, min, max|f|, sum f^2,
+// point count and a NaN flag, in a fixed tree order; the host combines the
+// blocks in block order.
+constexpr int kNormBlocks = 256;
+__global__ void __launch_bounds__(kPtBlock) k_norms_partial(KGrid G, const double* __restrict__ f,
+                                                            double* __restrict__ part) {
+  __shared__ double smax[kPtBlock], smin[kPtBlock], sabs[kPtBlock], ssq[kPtBlock], scnt[kPtBlock], snan[kPtBlock];
+  const int ni = G.Nr - G.g, nj = G.Nz - G.g;
+  const long n = static_cast<long>(ni) * nj;
+  double vmax = -1.0e300, vmin = 1.0e300, vabs = 0.0, sq = 0.0, cnt = 0.0, nan = 0.0;
+  for (long q = static_cast<long>(blockIdx.x) * kPtBlock + threadIdx.x; q < n;
+       q += static_cast<long>(gridDim.x) * kPtBlock) {
+    const int i = 1 + static_cast<int>(q / nj), j = 1 + static_cast<int>(q % nj);
+    const double v = f[IDX(G, i, j)];
+    if (v != v) nan = 1.0;
+    vmax = fmax(vmax, v);
+    vmin = fmin(vmin, v);
+    vabs = fmax(vabs, fabs(v));
+    sq += v * v;
+    cnt += 1.0;
+  }
+  const int t = static_cast<int>(threadIdx.x);
+  smax[t] = vmax; smin[t] = vmin; sabs[t] = vabs; ssq[t] = sq; scnt[t] = cnt; snan[t] = nan;
+  __syncthreads();
+  for (int h = kPtBlock / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      smax[t] = fmax(smax[t], smax[t + h]);
+      smin[t] = fmin(smin[t], smin[t + h]);
+      sabs[t] = fmax(sabs[t], sabs[t + h]);
+      ssq[t] += ssq[t + h];
+      scnt[t] += scnt[t + h];
+      snan[t] = fmax(snan[t], snan[t + h]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* o = part + 6 * blockIdx.x;
+    o[0] = smax[0]; o[1] = smin[0]; o[2] = sabs[0]; o[3] = ssq[0]; o[4] = scnt[0]; o[5] = snan[0];
+  }
+}
+
 // dtbeta = drivercsi * Delta (evolve.f90:174-195), whole arrays incl. images
 __global__ void k_dtbeta_init(long n, double csi, const double* __restrict__ dr, const double* __restrict__ dz,
                               double* __restrict__ tr, double* __restrict__ tz) {
@@ -368,6 +410,14 @@ int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void*
   hipLaunchKernelGGL(k_dtbeta_init, dim3(blocks_for(n)), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), n,
                      csi, y + S_Delta_r * pl, y + S_Delta_z * pl, y + shift_slot(SH_dtbeta_r) * pl,
                      y + shift_slot(SH_dtbeta_z) * pl);
+  return check_launch();
+}
+
+int norms_blocks() { return kNormBlocks; }
+
+int launch_norms_partial(const KGrid& G, const double* f, double* part, void* stream) {
+  hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), G, f,
+                     part);
   return check_launch();
 }
 

@@ -66,6 +66,13 @@ def param_text(p, key):
     return buf.value.decode()
 
 
+def footprint(params):
+    """Device bytes a Spacetime with these parameters allocates."""
+    v = ctypes.c_double()
+    _check(lib().oa_params_footprint(ctypes.byref(params), ctypes.byref(v)), "oa_params_footprint")
+    return v.value
+
+
 def _copy(p):
     q = Params()
     ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(Params))
@@ -174,6 +181,12 @@ class Spacetime:
 
     def advance_time(self, level):
         _check(lib().oa_advance_time(self._h, 0, level), "oa_advance_time")
+
+    def norms0d(self, name, level=0):
+        """save0Ddata's norms of an evolved array: dict(max, min, nm1, nm2)."""
+        out = (ctypes.c_double * 4)()
+        _check(lib().oa_norms0d(self._h, 0, level, name.encode(), out), "oa_norms0d " + name)
+        return dict(max=out[0], min=out[1], nm1=out[2], nm2=out[3])
 
     # timing
     def last_elapsed_ms(self):

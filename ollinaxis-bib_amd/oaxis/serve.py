@@ -1,0 +1,197 @@
+"""Serving loop: continuous batching of evolution requests on one device.
+
+Requests (par-file text, number of coarse steps, arrival time) are admitted
+as they arrive while their device footprint fits the memory budget and a
+member slot is free.  Every admitted spacetime keeps its state on the device
+and is advanced inside one Batch, so requests that arrive while others run
+join the same launches.  A request retires when it has done its coarse
+steps (the reference's Nt), when its lapse collapses (the build's stand-in
+for horizon formation, checked every ``check_every`` coarse steps: min alpha
+on the finest level falls below ``alpha_collapse``, for requests whose lapse
+starts above it) or when its scalar field has
+dispersed (max |scalar_phi| on level 0 below ``disperse_frac`` of its
+initial value).  The reference has no serving loop; the retirement criteria
+other than the step count are defined here.
+
+Completion latency is retirement time minus arrival time; run() returns one
+record per request and the aggregate throughput.
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Request:
+    rid: int
+    text: str                 # par-file text (reference keys)
+    steps: int                # coarse steps to run (Nt)
+    arrival: float = 0.0      # seconds after the start of run()
+    kind: str = ""
+
+
+@dataclass
+class Record:
+    rid: int
+    kind: str
+    arrival: float
+    admitted: float = -1.0
+    retired: float = -1.0
+    reason: str = ""
+    steps: int = 0
+    point_updates: float = 0.0
+    footprint: float = 0.0
+    norms: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.retired - self.arrival
+
+
+class Server:
+    def __init__(self, memory_budget, max_members=256, check_every=16, alpha_collapse=0.05,
+                 disperse_frac=0.01, on_retire=None):
+        import oaxis
+        self.oaxis = oaxis
+        self.budget = float(memory_budget)
+        self.max_members = int(max_members)
+        self.check_every = int(check_every)
+        self.alpha_collapse = alpha_collapse
+        self.disperse_frac = disperse_frac
+        self.on_retire = on_retire       # callback(request, spacetime, record) before the state is freed
+
+    def _admit(self, req, now, batch, active, used):
+        oaxis = self.oaxis
+        p = oaxis.params_from_text(req.text)
+        fp = oaxis.footprint(p)
+        st = oaxis.Spacetime(p)
+        st.initial_data()
+        st.initial_aux()
+        rec = Record(req.rid, req.kind, req.arrival, admitted=now, footprint=fp)
+        scalar = bool(p.mattertype & oaxis.ENUMS["OA_MATTER_SCALAR"])
+        phi0 = st.norms0d("scalar_phi", 0)["nm1"] if scalar else 0.0
+        # only a collapse that forms during the evolution retires a request
+        # (a puncture's lapse starts near zero)
+        watch = st.norms0d("alpha", st.nlevels - 1)["min"] >= self.alpha_collapse
+        batch.add(st)
+        active.append(dict(req=req, st=st, rec=rec, steps=0, scalar=scalar, phi0=phi0, watch=watch,
+                           u0=st.point_updates()))
+        return used + fp
+
+    def _retire(self, a, reason, now, batch):
+        st, rec = a["st"], a["rec"]
+        batch.remove(st)
+        rec.retired = now
+        rec.reason = reason
+        rec.steps = a["steps"]
+        rec.point_updates = st.point_updates() - a["u0"]
+        if reason != "failed":
+            try:
+                rec.norms = {"alpha": st.norms0d("alpha", 0), "chi": st.norms0d("chi", 0)}
+            except self.oaxis.OaxisError:
+                rec.reason = "failed"
+        if self.on_retire is not None:
+            self.on_retire(a["req"], st, rec)
+        st.close()
+        return rec
+
+    def _check(self, a):
+        st = a["st"]
+        lev = st.nlevels - 1
+        try:
+            if a["watch"] and st.norms0d("alpha", lev)["min"] < self.alpha_collapse:
+                return "lapse_collapse"
+            if a["scalar"] and a["phi0"] > 0.0:
+                if st.norms0d("scalar_phi", 0)["nm1"] < self.disperse_frac * a["phi0"]:
+                    return "dispersed"
+        except self.oaxis.OaxisError:
+            return "failed"          # a NaN in the state: the request retires with that status
+        return None
+
+    def run(self, requests, clock=time.perf_counter):
+        """Serve ``requests`` (arrival times relative to the start); returns
+        (records, summary)."""
+        oaxis = self.oaxis
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active, done = [], []
+        used = 0.0
+        batch = oaxis.Batch()
+        t0 = clock()
+        total_updates = 0.0
+        try:
+            while pending or active:
+                now = clock() - t0
+                # admit what has arrived, while memory and slots allow
+                while (pending and pending[0].arrival <= now and len(active) < self.max_members):
+                    fp = oaxis.footprint(oaxis.params_from_text(pending[0].text))
+                    if active and used + fp > self.budget:
+                        break
+                    if fp > self.budget:
+                        raise MemoryError("request %d needs %.3g B, more than the budget" % (pending[0].rid, fp))
+                    used = self._admit(pending.pop(0), now, batch, active, used)
+                    now = clock() - t0
+                if not active:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                    continue
+                batch.coarse_step()
+                for a in active:
+                    a["steps"] += 1
+                retire = []
+                for a in active:
+                    if a["steps"] >= a["req"].steps:
+                        retire.append((a, "final_time"))
+                    elif self.check_every > 0 and a["steps"] % self.check_every == 0:
+                        reason = self._check(a)
+                        if reason:
+                            retire.append((a, reason))
+                if retire:
+                    batch.sync()
+                    now = clock() - t0
+                    for a, reason in retire:
+                        rec = self._retire(a, reason, now, batch)
+                        total_updates += rec.point_updates
+                        used -= rec.footprint
+                        active.remove(a)
+                        done.append(rec)
+        finally:
+            for a in active:
+                a["st"].close()
+            batch.close()
+        wall = clock() - t0
+        lat = np.array([r.latency for r in done]) if done else np.zeros(1)
+        summary = dict(requests=len(done), wall_s=wall, point_updates=total_updates,
+                       updates_per_s=total_updates / wall if wall > 0 else 0.0,
+                       p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                       reasons={k: sum(1 for r in done if r.reason == k) for k in set(r.reason for r in done)})
+        return sorted(done, key=lambda r: r.rid), summary
+
+
+def serving_mix(seed, n, rate, grids=((64, 128), (128, 256), (256, 512), (512, 1024)), max_levels=6,
+                steps=(200, 2000)):
+    """Seeded request stream of the serving mix: Poisson arrivals at ``rate``
+    per second, 40 % Schwarzschild (M~U[.5,2]), 30 % Brill (a~U[1,4]), 30 %
+    scalar pulse; grid, level count (1..max_levels) and coarse steps drawn
+    uniformly."""
+    from . import workloads
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        u = rng.uniform()
+        Nr, Nz = grids[int(rng.integers(0, len(grids)))]
+        nl0 = int(rng.integers(0, max_levels))
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        if u < 0.4:
+            kind = "schwarzschild"
+            text = workloads.schwarzschild(float(rng.uniform(0.5, 2.0)), Nr, Nz, nl0)
+        elif u < 0.7:
+            kind = "brill"
+            text = workloads.brill(int(rng.integers(0, 2 ** 31)), Nr, Nz, nl0, dr0=25.6 / Nr)
+        else:
+            kind = "scalar_pulse"
+            text = workloads.scalar_pulse(seed, k) + "\nNrtotal = %d\nNztotal = %d\nNl0 = %d\ndr0 = %r\ndz0 = %r\n" % (
+                Nr, Nz, nl0, 12.0 / Nr, 12.0 / Nr)
+        out.append(Request(k, text, nsteps, t, kind))
+    return out

@@ -5,6 +5,7 @@ text drives the reference Fortran code, the oracle and this library.  Values
 drawn at random come from numpy.random.default_rng(seed).
 
   plain_small   Schwarzschild puncture M=1, 128x256, dr0=0.05, RK4, 100 steps
+  schwarzschild the same physics for any mass, grid and level count
   brill         Brill (Holz) wave a~U[1,4], 5 levels of 256x512, ICN,
                 1+log + Gamma-driver 2, 512 coarse steps
   scalar_pulse  one member of the batched scalar-collapse family
@@ -40,6 +41,16 @@ ahfind = .false.
 def plain_small(seed=0, Nrtotal=128, Nztotal=256):
     text = PLAIN_SMALL.replace("Nrtotal = 128", "Nrtotal = %d" % Nrtotal)
     return text.replace("Nztotal = 256", "Nztotal = %d" % Nztotal)
+
+
+def schwarzschild(mass, Nrtotal=128, Nztotal=256, Nl0=0, extent=6.4):
+    """The plain configuration's physics with mass ``mass`` on any grid,
+    rho in (0, extent] on the coarsest level."""
+    text = plain_small(0, Nrtotal, Nztotal)
+    dr = extent / Nrtotal
+    text = text.replace("BH1mass = 1.0", "BH1mass = %.17g" % mass)
+    text = text.replace("dr0 = 0.05", "dr0 = %.17g" % dr).replace("dz0 = 0.05", "dz0 = %.17g" % dr)
+    return text.replace("Nl0 = 0", "Nl0 = %d" % Nl0)
 
 
 def brill(seed=0, Nrtotal=256, Nztotal=512, Nl0=4, dr0=0.1):
