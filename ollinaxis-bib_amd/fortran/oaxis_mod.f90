@@ -21,7 +21,12 @@ module oaxis_mod
             oa_upload_field, oa_download_field, oa_get_time, oa_set_time, &
             oa_saveold, oa_substep, oa_finebound, oa_auxiliary, oa_restrict, oa_advance_time, &
             oa_initial_aux, oa_coarse_step, oa_evolve, oa_sync, oa_initial_data, &
-            oa_last_elapsed_ms, oa_point_updates
+            oa_last_elapsed_ms, oa_point_updates, oa_params_parse_text, oa_set_kernel_timing, &
+            oa_rhs_kernel_stats, oa_norms0d, oa_params_footprint, &
+            oa_batch_create, oa_batch_destroy, oa_batch_add, oa_batch_remove, oa_batch_size, &
+            oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
+            oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
+            oa_batch_rhs_kernel_stats
 
   ! storage for one oa_params block
   type :: oaxis_params
@@ -185,6 +190,97 @@ module oaxis_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double) :: u
+    end function
+    integer(c_int) function oa_params_parse_text(p, text) bind(C, name="oa_params_parse_text")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: p
+      character(kind=c_char), dimension(*) :: text
+    end function
+    integer(c_int) function oa_set_kernel_timing(h, on) bind(C, name="oa_set_kernel_timing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function oa_rhs_kernel_stats(h, avg_ms, launches) bind(C, name="oa_rhs_kernel_stats")
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: h
+      real(c_double) :: avg_ms
+      integer(c_long) :: launches
+    end function
+    ! 0D norms (max, min, nm1, nm2) of an evolved array, as save0Ddata forms them
+    integer(c_int) function oa_norms0d(h, box, level, name, norms) bind(C, name="oa_norms0d")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
+      character(kind=c_char), dimension(*) :: name
+      real(c_double), dimension(4) :: norms
+    end function
+    integer(c_int) function oa_params_footprint(p, bytes) bind(C, name="oa_params_footprint")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: p
+      real(c_double) :: bytes
+    end function
+    ! batches of spacetimes (no counterpart in the reference)
+    integer(c_int) function oa_batch_create(b) bind(C, name="oa_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr) :: b
+    end function
+    integer(c_int) function oa_batch_destroy(b) bind(C, name="oa_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function oa_batch_add(b, h) bind(C, name="oa_batch_add")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, h
+    end function
+    integer(c_int) function oa_batch_remove(b, h) bind(C, name="oa_batch_remove")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, h
+    end function
+    integer(c_int) function oa_batch_size(b, n) bind(C, name="oa_batch_size")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: n
+    end function
+    integer(c_int) function oa_batch_coarse_step(b) bind(C, name="oa_batch_coarse_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function oa_batch_evolve(b, nsteps) bind(C, name="oa_batch_evolve")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function oa_batch_sync(b) bind(C, name="oa_batch_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+    integer(c_int) function oa_batch_last_elapsed_ms(b, ms) bind(C, name="oa_batch_last_elapsed_ms")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      real(c_double) :: ms
+    end function
+    integer(c_int) function oa_batch_point_updates(b, u) bind(C, name="oa_batch_point_updates")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      real(c_double) :: u
+    end function
+    integer(c_int) function oa_batch_launch_stats(b, launches, items) bind(C, name="oa_batch_launch_stats")
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: b
+      integer(c_long) :: launches, items
+    end function
+    integer(c_int) function oa_batch_set_kernel_timing(b, on) bind(C, name="oa_batch_set_kernel_timing")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function oa_batch_rhs_kernel_stats(b, total_ms, launches, points) &
+        bind(C, name="oa_batch_rhs_kernel_stats")
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: b
+      real(c_double) :: total_ms, points
+      integer(c_long) :: launches
     end function
   end interface
 

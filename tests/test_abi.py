@@ -76,3 +76,29 @@ def test_create_without_device_fails_cleanly(oaxis_mod):
     p = oaxis_mod.params_from_text("order = four\nNrtotal = 16\nNztotal = 16\n")
     with pytest.raises(oaxis_mod.OaxisError, match="GPU"):
         oaxis_mod.Spacetime(p)
+
+
+def test_fortran_module_binds_every_declared_function(oaxis_mod):
+    """The ISO_C_BINDING module gives the Fortran main loop every entry point
+    of the C header, under the same name."""
+    import os
+    import re
+    from oaxis import _abi
+    src = open(os.path.join(os.path.dirname(_abi.HEADER), "..", "ollinaxis-bib_amd", "fortran",
+                            "oaxis_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(oa_\w+)"\)', src))
+    missing = [n for n in oaxis_mod._abi.declared_symbols() if n not in bound]
+    assert not missing, missing
+
+
+def test_fortran_module_exports_every_binding():
+    """Every bound entry point is public in the module."""
+    import os
+    import re
+    here = os.path.dirname(os.path.abspath(__file__))
+    src = open(os.path.join(here, "..", "ollinaxis-bib_amd", "fortran", "oaxis_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(oa_\w+)"\)', src))
+    public = set()
+    for block in re.findall(r"public ::((?:[^\n]*&\n)*[^\n]*)", src):
+        public |= set(re.findall(r"oa_\w+", block))
+    assert bound <= public, sorted(bound - public)
