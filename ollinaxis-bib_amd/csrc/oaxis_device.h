@@ -850,7 +850,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
       if (!last) a[k] = first ? S.w * s : a[k] + S.w * s;
       else sv = a[k] + S.w * s;
     }
-    const double base = first ? fld(slot)(i, j) : yp[slot * pl + k];
+    // y_p (on the first iteration the staged input is y_p itself)
+    const double base = yp[slot * pl + k];
     return base + S.dtw * sv;
   };
 

@@ -72,6 +72,19 @@ __device__ __forceinline__ void stage_tile(const KGrid& G, double* tile, int nst
   __syncthreads();
 }
 
+// Pointers read from a work item are generic; every one of them addresses
+// device (global) memory and is uniform across the block.  Passing each
+// through the global address space, behind an empty asm so the two casts do
+// not fold away, lets the compiler emit global loads and stores instead of
+// flat ones (flat accesses also count against the LDS wait counter, so every
+// LDS wait would also wait for outstanding global traffic).
+template <class T>
+__device__ __forceinline__ T* gmem(T* p) {
+  __attribute__((address_space(1))) T* g = (__attribute__((address_space(1))) T*)p;
+  asm volatile("" : "+s"(g));
+  return (T*)g;
+}
+
 // the item of block b in a table launch: the last one whose block0 <= b
 // (block0 ascends; the loads are uniform across the block)
 template <class Item>
@@ -96,14 +109,16 @@ template <bool SHIFT, bool SCALAR, bool PHIMODE>
 __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, double* tile) {
   constexpr int nf = nslots(SHIFT, SCALAR);
   const KGrid& G = it.G;
-  const KStep& S = it.S;
+  KStep S = it.S;
+  S.dtb_pre = gmem(S.dtb_pre);            // null stays null
   const int nbj = it.nbj, nbi = it.nbi;
   const int j0 = tile_start(bj, nbj, G.jlo, G.Nz);
   const int i0 = tile_start(bi, nbi, 1, G.Nr);
   const int ib = i0 - kHalo, jb = j0 - kHalo;
   // the evolved fields, then (first substep only) the dtbeta planes it differentiates
   const int nst = nf + (S.dtb_pre ? 2 : 0);
-  const double* __restrict__ yin = it.yin;
+  const double* __restrict__ yin = gmem(it.yin);
+  double* const gaux = gmem(it.gaux);
   stage_tile(G, tile, nst, [&](int f) { return (f < nf) ? yin + f * G.plane : S.dtb_pre + (f - nf) * G.plane; },
              ib, jb);
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
@@ -115,11 +130,11 @@ __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, doub
   // carrying phase-1 values or loads into phase 2, so the two phases do not
   // add up in registers
   if (interior_tile(G, i0, j0))
-    rhs_point<false, false, PHIMODE, true, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
+    rhs_point<false, false, PHIMODE, true, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
   else
-    rhs_point<false, false, PHIMODE, false, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
+    rhs_point<false, false, PHIMODE, false, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
   asm volatile("" ::: "memory");
-  rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, it.P, S, V, it.yp, it.acc, it.yout, it.gaux, i, j);
+  rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, it.P, S, V, gmem(it.yp), gmem(it.acc), gmem(it.yout), gaux, i, j);
 }
 
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
@@ -146,12 +161,12 @@ __device__ __forceinline__ bool level_point(const KGrid& G, long q, int& i, int&
 template <bool SHIFT, bool SCALAR>
 __device__ __forceinline__ void project_body(const ProjItem& it, long q) {
   int i, j;
-  if (level_point(it.G, q, i, j)) project_point<SHIFT, SCALAR>(it.G, it.P, it.time0, it.y, i, j);
+  if (level_point(it.G, q, i, j)) project_point<SHIFT, SCALAR>(it.G, it.P, it.time0, gmem(it.y), i, j);
 }
 
 __device__ __forceinline__ void deldef_body(const DelItem& it, long q) {
   int i, j;
-  if (level_point(it.G, q, i, j)) deldef_point(it.G, it.y, i, j);
+  if (level_point(it.G, q, i, j)) deldef_point(it.G, gmem(it.y), i, j);
 }
 
 // saveold for the fine-boundary strips: blocks of one field, then the next
@@ -164,7 +179,7 @@ __device__ __forceinline__ void saveold_body(const SaveItem& it, int blk) {
   const int bpf = (per + kPtBlock - 1) / kPtBlock;
   const int f = blk / bpf;
   const int q = (blk % bpf) * kPtBlock + static_cast<int>(threadIdx.x);
-  if (f < it.nf && q < per) saveold_point(it.G, it.y, it.rR, it.zL, it.zR, f, q);
+  if (f < it.nf && q < per) saveold_point(it.G, gmem(it.y), gmem(it.rR), gmem(it.zL), gmem(it.zR), f, q);
 }
 
 // finebound: blocks of one field group, then the next (the group stays
@@ -175,8 +190,9 @@ __device__ __forceinline__ void finebound_body(const FbItem& it, int blk) {
   const int grp = blk / bpg;
   const long q = static_cast<long>(blk % bpg) * kPtBlock + threadIdx.x;
   if (grp >= finebound_groups(SHIFT, SCALAR) || q >= it.n) return;
-  finebound_point<SHIFT, SCALAR>(it.Gf, it.Gc, it.P, it.T, it.time0, it.yc, it.yf, it.rR0, it.rR1, it.zL0, it.zL1,
-                                 it.zR0, it.zR1, it.table[q], grp);
+  finebound_point<SHIFT, SCALAR>(it.Gf, it.Gc, it.P, it.T, it.time0, gmem(it.yc), gmem(it.yf), gmem(it.rR0),
+                                 gmem(it.rR1), gmem(it.zL0), gmem(it.zL1), gmem(it.zR0), gmem(it.zR1),
+                                 gmem(it.table)[q], grp);
 }
 
 // restrict: blocks of one field, then the next; one thread per coarse point
@@ -189,7 +205,7 @@ __device__ __forceinline__ void restrict_body(const RsItem& it, int blk) {
   const long q = static_cast<long>(blk % bpf) * kPtBlock + threadIdx.x;
   if (s >= it.nf || q >= per) return;
   const int ic = static_cast<int>(q / nzt) + 1 - Gc.g, jc = static_cast<int>(q % nzt) + 1 - Gc.g;
-  restrict_point(it.Gf, Gc, s, it.interporder, it.yf, it.yc, ic, jc);
+  restrict_point(it.Gf, Gc, s, it.interporder, gmem(it.yf), gmem(it.yc), ic, jc);
 }
 
 template <bool SHIFT, bool SCALAR>
