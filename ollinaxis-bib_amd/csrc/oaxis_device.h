@@ -843,16 +843,26 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
 
   // ---- RK: accumulate (prl/arrays.pl:533-599) and update (:1282-1328)
   const bool first = (S.iter == 1), last = (S.iter == S.niter);
+  // y_p and the accumulator of every slot, loaded together before any store
+  // (so one memory round trip serves them all); on the first iteration the
+  // staged input is y_p itself
+  constexpr int NF = nslots(SHIFT, SCALAR);
+  double yv[NF], av[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) yv[q] = yp[q * pl + k];
+  const bool use_acc = S.rk4 && !first;
+  if (use_acc) {
+#pragma unroll
+    for (int q = 0; q < NF; ++q) av[q] = acc[q * pl + k];
+  }
   auto rk = [&](int slot, double s) -> double {
     double sv = s;
     if (S.rk4) {
       double* a = acc + slot * pl;
-      if (!last) a[k] = first ? S.w * s : a[k] + S.w * s;
-      else sv = a[k] + S.w * s;
+      if (!last) a[k] = first ? S.w * s : av[slot] + S.w * s;
+      else sv = av[slot] + S.w * s;
     }
-    // y_p (on the first iteration the staged input is y_p itself)
-    const double base = yp[slot * pl + k];
-    return base + S.dtw * sv;
+    return yv[slot] + S.dtw * sv;
   };
 
   const bool pt0 = (S.proj_t0 != 0);
