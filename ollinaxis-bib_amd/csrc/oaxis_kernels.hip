@@ -137,10 +137,21 @@ __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, doub
   rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, it.P, S, V, gmem(it.yp), gmem(it.acc), gmem(it.yout), gaux, i, j);
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8),
+// and each XCD has its own L2.  Renumbering the blocks so that every XCD
+// receives one contiguous range of tiles keeps neighbouring tiles, whose
+// halos overlap, in the same L2.  A bijection of [0, total).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int total) {
+  const int q = total / kXcds, r = total % kXcds;
+  const int x = b % kXcds, l = b / kXcds;
+  return x * q + (x < r ? x : r) + l;
+}
+
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
 __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs_table(const RhsItem* __restrict__ items, int n) {
   extern __shared__ double tile[];
-  const int b = static_cast<int>(blockIdx.x);
+  const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const RhsItem& it = find_item(items, n, b);
   const int t = b - it.block0;
   rhs_tile<SHIFT, SCALAR, PHIMODE>(it, t / it.nbj, t % it.nbj, tile);
