@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ollinaxis-bib_amd"))
 
 METRIC = "grid-point updates/s (FP64 RK4) batched vs plain, 1 GPU; p50/p99 solve latency"
+TUNE_STEPS = 16                # steps the schedule autotuner needs before it settles
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, TFLOP/s
 
@@ -140,9 +141,11 @@ def run_batched(args):
         sts.append(st)
     nfields = workloads.evolved_count(True, True)
     b = oaxis.Batch(sts)
-    if args.warmup > 0:
-        b.evolve(args.warmup)
-    b.set_kernel_timing(True)
+    if args.no_graphs:
+        b.set_graphs(False)
+    # warm-up, long enough for the schedule autotuner to finish (it times
+    # each schedule over steady-state steps)
+    b.evolve(max(args.warmup, TUNE_STEPS))
     b.sync()
     u0 = b.point_updates()
     t0 = time.perf_counter()
@@ -150,14 +153,22 @@ def run_batched(args):
     b.sync()
     wall = time.perf_counter() - t0
     updates = b.point_updates() - u0
-    rhs_ms, rhs_n, rhs_points = b.rhs_kernel_stats()
     launches, items = b.launch_stats()
     device_ms = b.last_elapsed_ms()
+    schedule = b.schedule()
+    # RHS launch durations over a second timed pass, launched one by one
+    b.set_kernel_timing(True)
+    b.evolve(args.steps)
+    b.sync()
+    rhs_ms, rhs_n, rhs_points = b.rhs_kernel_stats()
     b.close()
     # the plain path: every member alone, one after the other
     plain_updates = 0.0
     plain_wall = 0.0
     for st in sts:
+        if args.no_graphs:
+            st.set_graphs(False)
+        st.evolve(TUNE_STEPS)          # untimed: the member's own schedule autotuning
         p0 = st.point_updates()
         st.sync()
         t1 = time.perf_counter()
@@ -180,7 +191,7 @@ def run_batched(args):
         "unit": "grid-point updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": max(args.warmup, TUNE_STEPS),
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -195,7 +206,7 @@ def run_batched(args):
                    "integrator": "rk4", "point_updates": updates, "device_ms": device_ms,
                    "launches_per_coarse_step": launches, "work_items_per_coarse_step": items,
                    "plain_value": plain_value, "plain_ms_per_step": plain_wall / args.steps * 1e3,
-                   "batched_vs_plain": value / plain_value},
+                   "batched_vs_plain": value / plain_value, "schedule": schedule},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_rhs_table", "avg_launch_ms": rhs_ms / rhs_n if rhs_n else 0.0,
@@ -233,6 +244,7 @@ def main():
                     help="scale of the 200-2000 coarse steps per request in the serving sample")
     ap.add_argument("--serving-rate", type=float, default=20.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--serving-budget-gb", type=float, default=200.0, help="device memory for admitted requests")
+    ap.add_argument("--no-graphs", action="store_true", help="launch coarse steps kernel by kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -270,19 +282,27 @@ def main():
     st = oaxis.Spacetime(params)
     st.initial_data()
     st.initial_aux()
-    if args.warmup > 0:
-        st.evolve(args.warmup)
+    if args.no_graphs:
+        st.set_graphs(False)
+    st.evolve(max(args.warmup, TUNE_STEPS))
     u0 = st.point_updates()
-    st.set_kernel_timing(True)
     st.sync()
     t0 = time.perf_counter()
     st.evolve(args.steps)
     st.sync()
     wall = time.perf_counter() - t0
     updates = st.point_updates() - u0
-    rhs_ms, rhs_n = st.rhs_kernel_stats()
     device_ms = st.last_elapsed_ms()
     value = updates / wall
+    schedule = st.schedule()
+    # the RHS kernel's launch durations: HIP events around each launch over a
+    # second timed pass of the same number of steps (launched one by one,
+    # since events cannot bracket kernels inside a graph)
+    st.set_kernel_timing(True)
+    st.evolve(args.steps)
+    st.sync()
+    rhs_ms, rhs_n = st.rhs_kernel_stats()
+    st.set_kernel_timing(False)
 
     bpu, niter = bytes_per_update(params, nfields)
     # algorithmic bytes per RHS launch: per-update bytes over the substeps of a
@@ -297,7 +317,7 @@ def main():
         "unit": "grid-point updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": max(args.warmup, TUNE_STEPS),
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -307,7 +327,7 @@ def main():
         "config": {"workload": desc, "path": "plain (one spacetime)", "levels": st.nlevels,
                    "Nrtotal": params.Nrtotal, "Nztotal": params.Nztotal, "evolved_fields": nfields,
                    "integrator": "rk4" if params.integrator == 1 else "icn",
-                   "point_updates": updates, "device_ms": device_ms},
+                   "point_updates": updates, "device_ms": device_ms, "schedule": schedule},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
                      "kernel": "k_rhs_table", "avg_launch_ms": rhs_ms, "launches": rhs_n,

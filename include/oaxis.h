@@ -218,8 +218,22 @@ int oa_initial_data(oa_handle* h);
 /* Device time of the last oa_evolve / oa_coarse_step call in ms (HIP events
  * on the handle's stream), and the kernel launch counts. */
 int oa_last_elapsed_ms(const oa_handle* h, double* ms);
+/* Schedule of whole coarse steps (oa_coarse_step, oa_evolve).  No
+ * counterpart in the reference.  A step runs either as a captured HIP graph
+ * (reused while the step's sequence of launches repeats, with fresh work
+ * items each step) or kernel by kernel, and the RHS blocks are either
+ * renumbered per XCD or not.  Neither choice changes any arithmetic, so by
+ * default an autotuner times each of the four schedules over steady-state
+ * steps (one untimed and two timed steps each) and keeps the fastest.
+ * oa_set_graphs fixes the first choice and turns the autotuner off.
+ * oa_schedule reports the schedule in use, whether tuning has finished, and
+ * the mean step time (ms) it measured for schedules 0..3 (bit 0: graph,
+ * bit 1: XCD renumbering). */
+int oa_set_graphs(oa_handle* h, int on);
+int oa_schedule(const oa_handle* h, int* graphs, int* xcd_remap, int* tuned, double* step_ms);
 /* Per-launch HIP-event timing of the dominant RHS kernel during oa_evolve
- * (off by default; adds two event records per launch on the same stream). */
+ * (off by default; adds two event records per launch on the same stream;
+ * while on, steps are launched directly rather than as graphs). */
 int oa_set_kernel_timing(oa_handle* h, int on);
 /* Average duration (ms) of the dominant RHS kernel over the last timed
  * interval and how many launches it averaged. */
@@ -274,6 +288,9 @@ int oa_batch_launch_stats(const oa_batch* b, long* launches, long* items);
  * default): total kernel time (ms), launches, and grid points (Nr x Nz of
  * each item) those launches advanced by one substep. */
 int oa_batch_set_kernel_timing(oa_batch* b, int on);
+/* As oa_set_graphs / oa_schedule, for the batch's coarse steps. */
+int oa_batch_set_graphs(oa_batch* b, int on);
+int oa_batch_schedule(const oa_batch* b, int* graphs, int* xcd_remap, int* tuned, double* step_ms);
 int oa_batch_rhs_kernel_stats(const oa_batch* b, double* total_ms, long* launches, double* points);
 
 #ifdef __cplusplus

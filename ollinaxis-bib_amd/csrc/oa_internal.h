@@ -137,10 +137,14 @@ int rs_blocks(const RsItem& it);
 // than 16 points
 int rhs_tiles(const KGrid& G, int* nbi, int* nbj);
 
+// raise the LDS limits of the RHS kernels (once, before any launch)
+int prepare_rhs_kernels();
+
 // launchers (oaxis_kernels.hip): n items in device memory whose block0
 // fields are the prefix sums of their blocks, and the total
+// xcd_remap: renumber blocks so that each XCD takes one contiguous range of tiles
 int launch_rhs_table(const RhsItem* items, int n, int blocks, bool shift, bool scalar, bool phimode,
-                     bool any_dtb_pre, void* stream);
+                     bool any_dtb_pre, bool xcd_remap, void* stream);
 int launch_project_table(const ProjItem* items, int n, int blocks, bool shift, bool scalar, void* stream);
 int launch_deldef_table(const DelItem* items, int n, int blocks, void* stream);
 int launch_saveold_table(const SaveItem* items, int n, int blocks, void* stream);

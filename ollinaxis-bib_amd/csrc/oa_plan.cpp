@@ -31,13 +31,13 @@ int hip_check(hipError_t e, const char* what) {
     if (rc_ != OA_OK) return rc_;                     \
   } while (0)
 
-int issue(const LaunchRec& r, const char* dev, hipStream_t st) {
+int issue(const LaunchRec& r, const char* dev, bool xcd, hipStream_t st) {
   const char* base = dev + r.offset;
   switch (r.kind) {
     case K_SAVE: return launch_saveold_table(reinterpret_cast<const SaveItem*>(base), r.n, r.blocks, st);
     case K_RHS:
       return launch_rhs_table(reinterpret_cast<const RhsItem*>(base), r.n, r.blocks, r.shift, r.scalar, r.phimode,
-                              r.any_pre, st);
+                              r.any_pre, xcd, st);
     case K_FB:
       return launch_finebound_table(reinterpret_cast<const FbItem*>(base), r.n, r.blocks, r.shift, r.scalar, st);
     case K_DEL: return launch_deldef_table(reinterpret_cast<const DelItem*>(base), r.n, r.blocks, st);
@@ -54,8 +54,35 @@ int group_code(bool shift, bool scalar) { return (shift ? 2 : 0) + (scalar ? 1 :
 int Plan::init() {
   for (int k = 0; k < kSlots; ++k)
     HIPCHK(hipEventCreateWithFlags(&copied[k], hipEventDisableTiming));
-  return OA_OK;
+  for (GraphSlot& g : gslot) HIPCHK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+  HIPCHK(hipEventCreate(&tuner.t0));
+  HIPCHK(hipEventCreate(&tuner.t1));
+  return prepare_rhs_kernels();
 }
+
+namespace {
+
+bool same_launches(const std::vector<LaunchRec>& a, const std::vector<LaunchRec>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t q = 0; q < a.size(); ++q) {
+    const LaunchRec &x = a[q], &y = b[q];
+    if (x.kind != y.kind || x.shift != y.shift || x.scalar != y.scalar || x.phimode != y.phimode ||
+        x.offset != y.offset || x.n != y.n || x.blocks != y.blocks || x.any_pre != y.any_pre)
+      return false;
+  }
+  return true;
+}
+
+void drop_graph(Plan::GraphSlot& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+  g.sig.clear();
+  g.bytes = 0;
+}
+
+}  // namespace
 
 void Plan::release() {
   for (int k = 0; k < kSlots; ++k) {
@@ -65,6 +92,15 @@ void Plan::release() {
     copied[k] = nullptr;
     pinned_cap[k] = 0;
   }
+  for (GraphSlot& g : gslot) {
+    drop_graph(g);
+    if (g.pinned) (void)hipHostFree(g.pinned);
+    if (g.done) (void)hipEventDestroy(g.done);
+    g = GraphSlot{};
+  }
+  if (tuner.t0) (void)hipEventDestroy(tuner.t0);
+  if (tuner.t1) (void)hipEventDestroy(tuner.t1);
+  tuner.t0 = tuner.t1 = nullptr;
   if (dev) (void)hipFree(dev);
   dev = nullptr;
   dev_cap = 0;
@@ -72,9 +108,106 @@ void Plan::release() {
   kev.clear();
 }
 
-int Plan::submit(hipStream_t stream) {
+int Plan::submit(hipStream_t stream, bool as_graph) {
   if (launches.empty()) return OA_OK;
+  Tuner& t = tuner;
+  if (!as_graph || !t.enabled || t.done || kernel_timing) return submit_once(stream, as_graph);
+  if (t.rep == -2 || !same_launches(t.sig, launches)) {
+    // wait for a step that repeats the previous one
+    const bool repeat = (t.rep == -2) && same_launches(t.sig, launches);
+    t.sig = launches;
+    if (!repeat) {
+      t.rep = -2;
+      return submit_once(stream, as_graph);
+    }
+    t.cand = 0;
+    t.rep = -1;
+    t.best_cand = -1;
+  }
+  graphs = (t.cand & 1) != 0;
+  xcd_remap = (t.cand & 2) != 0;
+  HIPCHK(hipEventRecord(t.t0, stream));
+  RC(submit_once(stream, as_graph));
+  HIPCHK(hipEventRecord(t.t1, stream));
+  HIPCHK(hipEventSynchronize(t.t1));
+  float m = 0.0f;
+  HIPCHK(hipEventElapsedTime(&m, t.t0, t.t1));
+  if (t.rep >= 0) t.acc += m;
+  if (++t.rep == kTuneReps) {
+    t.ms[t.cand] = t.acc / kTuneReps;
+    if (t.best_cand < 0 || t.ms[t.cand] < t.best) {
+      t.best = t.ms[t.cand];
+      t.best_cand = t.cand;
+    }
+    t.acc = 0.0;
+    t.rep = -1;
+    if (++t.cand == kTuneCands) {
+      t.done = true;
+      graphs = (t.best_cand & 1) != 0;
+      xcd_remap = (t.best_cand & 2) != 0;
+    }
+  }
+  return OA_OK;
+}
+
+int Plan::submit_once(hipStream_t stream, bool as_graph) {
   const size_t bytes = arena.size();
+  if (as_graph && graphs && !kernel_timing) {
+    GraphSlot& g = gslot[gnext];
+    // this slot's pinned buffer is free once its previous launch has finished
+    if (g.pending) {
+      HIPCHK(hipEventSynchronize(g.done));
+      g.pending = false;
+    }
+    if (bytes > dev_cap) {
+      // a larger device arena: every captured graph copies to the old one
+      if (dev) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipFree(dev));
+      }
+      dev = nullptr;
+      dev_cap = 0;
+      const size_t cap = std::max(bytes * 2, static_cast<size_t>(64) << 10);
+      HIPCHK(hipMalloc(reinterpret_cast<void**>(&dev), cap));
+      dev_cap = cap;
+    }
+    if (bytes > g.pinned_cap) {
+      drop_graph(g);
+      if (g.pinned) HIPCHK(hipHostFree(g.pinned));
+      g.pinned = nullptr;
+      g.pinned_cap = 0;
+      const size_t cap = std::max(bytes * 2, static_cast<size_t>(64) << 10);
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&g.pinned), cap, hipHostMallocDefault));
+      g.pinned_cap = cap;
+    }
+    if (!g.exec || g.dev != dev || g.bytes != bytes || g.xcd != xcd_remap || !same_launches(g.sig, launches)) {
+      drop_graph(g);
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      int rc = hip_check(hipMemcpyAsync(dev, g.pinned, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+      for (size_t q = 0; rc == OA_OK && q < launches.size(); ++q) rc = issue(launches[q], dev, xcd_remap, stream);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(stream, &graph);
+      if (rc != OA_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(e);
+      g.graph = graph;
+      HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+      g.sig = launches;
+      g.bytes = bytes;
+      g.dev = dev;
+      g.xcd = xcd_remap;
+      ++graph_captures;
+    }
+    std::memcpy(g.pinned, arena.data(), bytes);
+    HIPCHK(hipGraphLaunch(g.exec, stream));
+    HIPCHK(hipEventRecord(g.done, stream));
+    g.pending = true;
+    gnext ^= 1;
+    ++graph_launches;
+    return OA_OK;
+  }
   const int k = slot;
   // the pinned buffer of this slot is free once its last transfer is done
   if (copied_pending[k]) {
@@ -120,7 +253,7 @@ int Plan::submit(hipStream_t stream) {
       }
       HIPCHK(hipEventRecord(kev[kev_used], stream));
     }
-    RC(issue(r, dev, stream));
+    RC(issue(r, dev, xcd_remap, stream));
     if (timed) {
       HIPCHK(hipEventRecord(kev[kev_used + 1], stream));
       kev_points[kev_used / 2] = r.points;

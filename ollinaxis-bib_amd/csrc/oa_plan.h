@@ -50,7 +50,48 @@ struct Plan {
   int slot = 0;
   char* dev = nullptr;
   size_t dev_cap = 0;
-  // optional HIP-event timing of the RHS launches
+  // Whole coarse steps can run as captured HIP graphs: the transfer of the
+  // arena plus every launch.  A graph is reused while the sequence of
+  // launches (kinds, variants, item counts, grids, arena layout) repeats,
+  // which it does from step to step once every level's clock is past 0; only
+  // the work items change, and they reach the device through the graph's own
+  // transfer.  Two graphs alternate, each with its own pinned source buffer,
+  // so the host can prepare one step while the previous one runs.
+  struct GraphSlot {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<LaunchRec> sig;
+    size_t bytes = 0;
+    char* pinned = nullptr;
+    size_t pinned_cap = 0;
+    char* dev = nullptr;          // device arena the graph copies to
+    bool xcd = false;             // the RHS block numbering it was captured with
+    hipEvent_t done = nullptr;    // recorded after the graph's last launch
+    bool pending = false;
+  };
+  bool graphs = true;
+  bool xcd_remap = true;          // RHS blocks renumbered per XCD
+  GraphSlot gslot[2];
+  // Schedule autotuner.  The schedule of a coarse step — launched as a graph
+  // or kernel by kernel, RHS blocks renumbered per XCD or not — does not
+  // change any arithmetic (the same kernels run on the same tiles and items),
+  // so it is chosen by measurement: once the sequence of launches repeats,
+  // each of the four schedules runs kTuneReps timed steps (after one untimed
+  // step, which pays for a graph capture), and the fastest is kept for the
+  // rest of the plan's life.
+  static constexpr int kTuneCands = 4, kTuneReps = 2;
+  struct Tuner {
+    bool enabled = true, done = false;
+    std::vector<LaunchRec> sig;   // the repeating step being timed
+    int cand = 0, rep = -2;       // rep -2: waiting for the step to repeat; -1: untimed step
+    double acc = 0.0, best = 0.0;
+    int best_cand = 0;
+    double ms[kTuneCands] = {};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+  } tuner;
+  int gnext = 0;
+  long graph_launches = 0, graph_captures = 0;
+  // optional HIP-event timing of the RHS launches (direct launches only)
   bool kernel_timing = false;
   std::vector<hipEvent_t> kev;
   std::vector<double> kev_points;
@@ -88,7 +129,10 @@ struct Plan {
     std::vector<Item> v(1, it);
     add(kind, shift, scalar, phimode, v);
   }
-  int submit(hipStream_t stream);   // transfer the arena, enqueue the launches
+  // transfer the arena and enqueue the launches; as_graph: a whole coarse
+  // step that may run as a captured graph
+  int submit(hipStream_t stream, bool as_graph = false);
+  int submit_once(hipStream_t stream, bool as_graph);
   // RHS timing over the launches since the last reset
   void reset_timing() { kev_used = 0; }
   int rhs_stats(double* total_ms, long* launches, double* points) const;

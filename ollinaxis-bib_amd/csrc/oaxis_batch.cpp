@@ -74,7 +74,7 @@ int batch_step(oa_batch* b) {
   b->point_updates += after - before;
   b->last_launches = static_cast<long>(b->plan.launches.size());
   b->last_items = b->plan.items();
-  return b->plan.submit(b->stream);
+  return b->plan.submit(b->stream, true);
 }
 
 bool member_of(const oa_batch* b, const oa_handle* h) {
@@ -163,6 +163,24 @@ extern "C" int oa_batch_evolve(oa_batch* b, int nsteps) {
   HIPCHK(hipEventElapsedTime(&ms, b->ev_begin, b->ev_end));
   b->last_ms = ms;
   if (b->plan.kernel_timing) RC(b->plan.rhs_stats(&b->rhs_ms, &b->rhs_launches, &b->rhs_points));
+  return OA_OK;
+}
+
+extern "C" int oa_batch_set_graphs(oa_batch* b, int on) {
+  if (!b) return OA_ERR_ARG;
+  b->plan.graphs = (on != 0);
+  b->plan.tuner.enabled = false;     // the caller's choice stands
+  return OA_OK;
+}
+
+extern "C" int oa_batch_schedule(const oa_batch* b, int* graphs, int* xcd_remap, int* tuned, double* step_ms) {
+  if (!b) return OA_ERR_ARG;
+  const Plan& pl = b->plan;
+  if (graphs) *graphs = pl.graphs ? 1 : 0;
+  if (xcd_remap) *xcd_remap = pl.xcd_remap ? 1 : 0;
+  if (tuned) *tuned = pl.tuner.done ? 1 : 0;
+  if (step_ms)
+    for (int c = 0; c < Plan::kTuneCands; ++c) step_ms[c] = pl.tuner.ms[c];
   return OA_OK;
 }
 

@@ -326,7 +326,7 @@ int do_restrict(oa_handle* h, int level) {
 int coarse_step(oa_handle* h) {
   h->plan.begin();
   RC(plan_level(h->plan, std::vector<oa_handle*>{h}, 0));
-  return h->plan.submit(h->stream);
+  return h->plan.submit(h->stream, true);
 }
 
 void fill_grid(KGrid& G, const oa_params& p, int level) {
@@ -762,6 +762,24 @@ extern "C" int oa_sync(oa_handle* h) {
 extern "C" int oa_last_elapsed_ms(const oa_handle* h, double* ms) {
   if (!h || !ms) return OA_ERR_ARG;
   *ms = h->last_ms;
+  return OA_OK;
+}
+
+extern "C" int oa_set_graphs(oa_handle* h, int on) {
+  if (!h) return OA_ERR_ARG;
+  h->plan.graphs = (on != 0);
+  h->plan.tuner.enabled = false;     // the caller's choice stands
+  return OA_OK;
+}
+
+extern "C" int oa_schedule(const oa_handle* h, int* graphs, int* xcd_remap, int* tuned, double* step_ms) {
+  if (!h) return OA_ERR_ARG;
+  const Plan& pl = h->plan;
+  if (graphs) *graphs = pl.graphs ? 1 : 0;
+  if (xcd_remap) *xcd_remap = pl.xcd_remap ? 1 : 0;
+  if (tuned) *tuned = pl.tuner.done ? 1 : 0;
+  if (step_ms)
+    for (int c = 0; c < Plan::kTuneCands; ++c) step_ms[c] = pl.tuner.ms[c];
   return OA_OK;
 }
 

@@ -149,9 +149,11 @@ __device__ __forceinline__ int xcd_block(int b, int total) {
 }
 
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
-__global__ void __launch_bounds__(kTile * kTile, 2) k_rhs_table(const RhsItem* __restrict__ items, int n) {
+__global__ void __launch_bounds__(kTile * kTile, 2) k_rhs_table(const RhsItem* __restrict__ items, int n,
+                                                                int xcd_remap) {
   extern __shared__ double tile[];
-  const int b = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int b = xcd_remap ? xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))
+                          : static_cast<int>(blockIdx.x);
   const RhsItem& it = find_item(items, n, b);
   const int t = b - it.block0;
   rhs_tile<SHIFT, SCALAR, PHIMODE>(it, t / it.nbj, t % it.nbj, tile);
@@ -337,8 +339,11 @@ int save_blocks(const SaveItem& it) {
   const int nrt = it.G.Nr + it.G.g, nzt = it.G.Nz + it.G.g;
   return it.nf * blocks_for(static_cast<long>(it.G.g) * (nrt > nzt ? nrt : nzt));
 }
+// Sized for the deeper of the two zones whichever this call fills, so that
+// the grid of a finebound launch does not depend on whether the clocks meet
+// (a floating comparison); the blocks past the item's own records exit.
 int fb_blocks(const FbItem& it, bool shift, bool scalar) {
-  return finebound_groups(shift, scalar) * blocks_for(it.n);
+  return finebound_groups(shift, scalar) * blocks_for(fb_count(it.Gf, it.Gf.g + 2));
 }
 int rs_blocks(const RsItem& it) {
   return it.nf * blocks_for(static_cast<long>(it.Gc.Nr + it.Gc.g) * (it.Gc.Nz + it.Gc.g));
@@ -369,11 +374,29 @@ static int rhs_lds(bool dtb_pre, size_t* lds) {
 }
 
 template <bool SH, bool SC, bool PM>
-static int launch_rhs_table_t(const RhsItem* items, int n, int blocks, bool any_pre, hipStream_t st) {
+static int launch_rhs_table_t(const RhsItem* items, int n, int blocks, bool any_pre, bool xcd, hipStream_t st) {
   size_t lds = 0;
   RC_K((rhs_lds<SH, SC, PM>(any_pre, &lds)));
-  hipLaunchKernelGGL((k_rhs_table<SH, SC, PM>), dim3(blocks), dim3(kTile, kTile, 1), lds, st, items, n);
+  hipLaunchKernelGGL((k_rhs_table<SH, SC, PM>), dim3(blocks), dim3(kTile, kTile, 1), lds, st, items, n, xcd ? 1 : 0);
   return check_launch();
+}
+
+// The LDS limits of every RHS variant, raised once before any launch (and so
+// outside any graph capture).
+int prepare_rhs_kernels() {
+  static int rc = -1;
+  if (rc >= 0) return rc;
+  size_t lds = 0;
+  rc = OA_OK;
+  if (rc == OA_OK) rc = rhs_lds<false, false, false>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<false, false, true>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<false, true, false>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<false, true, true>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<true, false, false>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<true, false, true>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<true, true, false>(false, &lds);
+  if (rc == OA_OK) rc = rhs_lds<true, true, true>(false, &lds);
+  return rc;
 }
 
 #define RHS_DISPATCH(FN, ...)                                   \
@@ -389,9 +412,9 @@ static int launch_rhs_table_t(const RhsItem* items, int n, int blocks, bool any_
   }
 
 int launch_rhs_table(const RhsItem* items, int n, int blocks, bool shift, bool scalar, bool phimode,
-                     bool any_dtb_pre, void* stream) {
+                     bool any_dtb_pre, bool xcd_remap, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  RHS_DISPATCH(launch_rhs_table_t, items, n, blocks, any_dtb_pre, st)
+  RHS_DISPATCH(launch_rhs_table_t, items, n, blocks, any_dtb_pre, xcd_remap, st)
 }
 
 #define SC_DISPATCH(K, GRID, ...)                                                                  \

@@ -26,7 +26,7 @@ module oaxis_mod
             oa_batch_create, oa_batch_destroy, oa_batch_add, oa_batch_remove, oa_batch_size, &
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
-            oa_batch_rhs_kernel_stats
+            oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule
 
   ! storage for one oa_params block
   type :: oaxis_params
@@ -195,6 +195,29 @@ module oaxis_mod
       import :: c_int, c_ptr, c_char
       type(c_ptr), value :: p
       character(kind=c_char), dimension(*) :: text
+    end function
+    integer(c_int) function oa_set_graphs(h, on) bind(C, name="oa_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function oa_batch_set_graphs(b, on) bind(C, name="oa_batch_set_graphs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function oa_schedule(h, graphs, xcd_remap, tuned, step_ms) bind(C, name="oa_schedule")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int) :: graphs, xcd_remap, tuned
+      real(c_double), dimension(4) :: step_ms
+    end function
+    integer(c_int) function oa_batch_schedule(b, graphs, xcd_remap, tuned, step_ms) &
+        bind(C, name="oa_batch_schedule")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int) :: graphs, xcd_remap, tuned
+      real(c_double), dimension(4) :: step_ms
     end function
     integer(c_int) function oa_set_kernel_timing(h, on) bind(C, name="oa_set_kernel_timing")
       import :: c_int, c_ptr

@@ -73,6 +73,15 @@ def footprint(params):
     return v.value
 
 
+def _schedule(fn, h):
+    """The coarse-step schedule in use and what the autotuner measured."""
+    g, x, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    ms = (ctypes.c_double * 4)()
+    _check(fn(h, ctypes.byref(g), ctypes.byref(x), ctypes.byref(t), ms), "schedule")
+    return dict(graphs=bool(g.value), xcd_remap=bool(x.value), tuned=bool(t.value),
+                step_ms={"direct": ms[0], "graph": ms[1], "direct+xcd": ms[2], "graph+xcd": ms[3]})
+
+
 def _copy(p):
     q = Params()
     ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(Params))
@@ -197,6 +206,12 @@ class Spacetime:
     def set_kernel_timing(self, on):
         _check(lib().oa_set_kernel_timing(self._h, 1 if on else 0), "oa_set_kernel_timing")
 
+    def set_graphs(self, on):
+        _check(lib().oa_set_graphs(self._h, 1 if on else 0), "oa_set_graphs")
+
+    def schedule(self):
+        return _schedule(lib().oa_schedule, self._h)
+
     def rhs_kernel_stats(self):
         ms, n = ctypes.c_double(), ctypes.c_long()
         _check(lib().oa_rhs_kernel_stats(self._h, ctypes.byref(ms), ctypes.byref(n)), "oa_rhs_kernel_stats")
@@ -259,6 +274,12 @@ class Batch:
 
     def set_kernel_timing(self, on):
         _check(lib().oa_batch_set_kernel_timing(self._b, 1 if on else 0), "oa_batch_set_kernel_timing")
+
+    def set_graphs(self, on):
+        _check(lib().oa_batch_set_graphs(self._b, 1 if on else 0), "oa_batch_set_graphs")
+
+    def schedule(self):
+        return _schedule(lib().oa_batch_schedule, self._b)
 
     def rhs_kernel_stats(self):
         """(total ms, launches, grid points advanced) of the RHS launches of

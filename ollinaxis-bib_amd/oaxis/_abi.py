@@ -121,6 +121,10 @@ def load_library(path=LIBPATH):
         "oa_batch_point_updates": (ctypes.c_int, [H, dp]),
         "oa_batch_launch_stats": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
         "oa_batch_set_kernel_timing": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_batch_set_graphs": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_set_graphs": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_schedule": (ctypes.c_int, [H, ip, ip, ip, dp]),
+        "oa_batch_schedule": (ctypes.c_int, [H, ip, ip, ip, dp]),
         "oa_batch_rhs_kernel_stats": (ctypes.c_int, [H, dp, ctypes.POINTER(ctypes.c_long), dp]),
     }
     for name, (res, args) in sig.items():

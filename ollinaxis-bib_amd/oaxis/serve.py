@@ -117,6 +117,9 @@ class Server:
         active, done = [], []
         used = 0.0
         batch = oaxis.Batch()
+        # the membership changes too often for the schedule autotuner to
+        # settle: a fixed schedule (graphs; RHS blocks renumbered per XCD)
+        batch.set_graphs(True)
         t0 = clock()
         total_updates = 0.0
         try:

@@ -116,3 +116,58 @@ def test_batch_join_and_leave(oaxis_mod):
     _assert_same(nc, _state(oaxis, c), ref[nc])
     for st in (a, bb, c):
         st.close()
+
+
+def test_graph_steps_equal_direct_launches(oaxis_mod):
+    """Coarse steps replayed from captured HIP graphs (reused across steps,
+    with fresh work items each step) equal steps launched kernel by kernel,
+    for a spacetime alone and for a batch."""
+    oaxis = oaxis_mod
+    members = _members(oaxis)[:4]
+    nsteps = 5
+    direct = {}
+    for name, p in members:
+        st = _fresh(oaxis, p)
+        st.set_graphs(False)
+        st.evolve(nsteps)
+        direct[name] = _state(oaxis, st)
+        st.close()
+    for name, p in members[:2]:
+        st = _fresh(oaxis, p)
+        st.set_graphs(True)
+        for _ in range(nsteps):
+            st.coarse_step()
+        _assert_same("graph " + name, _state(oaxis, st), direct[name])
+        st.close()
+    sts = [(name, _fresh(oaxis, p)) for name, p in members]
+    with oaxis.Batch([st for _, st in sts]) as b:
+        b.set_graphs(True)
+        b.evolve(2)
+        b.set_graphs(False)
+        b.evolve(1)
+        b.set_graphs(True)
+        b.evolve(2)
+        for name, st in sts:
+            _assert_same("batched graph " + name, _state(oaxis, st), direct[name])
+    for _, st in sts:
+        st.close()
+
+
+def test_schedule_autotuner_keeps_results(oaxis_mod):
+    """The schedule autotuner (graph or direct launches, XCD block numbering)
+    settles within 16 steady-state steps and changes no result."""
+    oaxis = oaxis_mod
+    name, p = _members(oaxis)[0]
+    nsteps = 20
+    ref = _fresh(oaxis, p)
+    ref.set_graphs(False)
+    ref.evolve(nsteps)
+    want = _state(oaxis, ref)
+    ref.close()
+    st = _fresh(oaxis, p)
+    st.evolve(nsteps)
+    sched = st.schedule()
+    assert sched["tuned"], sched
+    assert all(v > 0.0 for v in sched["step_ms"].values()), sched
+    _assert_same("autotuned " + name, _state(oaxis, st), want)
+    st.close()
