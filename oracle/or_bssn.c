@@ -464,6 +464,51 @@ void or_auxiliary_geometry(const oa_params* p, or_level* L) {
 }
 
 /* sources_geometry: sources_geometry.f90:77-575 (bssn, nolambda, !evolveKTH) */
+/* constraints(box, level) (src/geometry/constraints.f90:36-206), vacuum
+ * or real scalar field, hdet = 1, no angmom: the Hamiltonian, momentum and
+ * Delta constraints from the arrays of the last auxiliary call.  The
+ * routine's smallpi = acos(-1.0) (:41) takes a default-kind (single precision)
+ * argument, so pi there is the single-precision value. */
+void or_constraints(const oa_params* p, or_level* L) {
+  const int matter = (p->mattertype & OA_MATTER_SCALAR) != 0;
+  const double half = 0.5, two = 2.0, third = 1.0 / 3.0;
+  const double smallpi = (double)acosf(-1.0f);
+  for (int k = 0; k < L->n; ++k) {
+    const double r = L->r[k], tk = EV(trK)[k];
+    double h = L->RSCAL[k] - L->KT2[k] + two * third * tk * tk;                  /* :57 */
+    if (matter) h = h - 16.0 * smallpi * L->rho[k];                            /* :60 */
+    L->ham[k] = h;
+    double mr = L->DIV_KT_r[k] + 6.0 * (L->KT_up_rr[k] * L->Dr_phi[k] + L->KT_up_rz[k] * L->Dz_phi[k])
+                - two * third * (L->g_A[k] * L->Dr_trK[k] + r * L->g_C[k] * L->Dz_trK[k]);  /* :116-117, matter :120 */
+    double mz = L->DIV_KT_z[k] + 6.0 * (L->KT_up_rz[k] * L->Dr_phi[k] + L->KT_up_zz[k] * L->Dz_phi[k])
+                - two * third * (r * L->g_C[k] * L->Dr_trK[k] + L->g_B[k] * L->Dz_trK[k]);  /* :129-130, matter :133 */
+    if (matter) {
+      mr = mr - 8.0 * smallpi * L->J_r[k] * L->psi4[k];
+      mz = mz - 8.0 * smallpi * L->J_z[k] * L->psi4[k];
+    }
+    L->mom_r[k] = mr;
+    L->mom_z[k] = mz;
+    L->CDelta_r[k] = EV(Delta_r)[k] + (L->Dr_g_A[k] + r * L->Dz_g_C[k] + r * L->g_lambda[k]
+                     + half * (L->g_A[k] * L->Dr_hdet[k] + r * L->g_C[k] * L->Dz_hdet[k]) * L->ihdet[k]);  /* :165-166 */
+    L->CDelta_z[k] = EV(Delta_z)[k] + (L->Dr_g_C[k] * r + L->Dz_g_B[k] + two * L->g_C[k]
+                     + half * (r * L->g_C[k] * L->Dr_hdet[k] + L->g_B[k] * L->Dz_hdet[k]) * L->ihdet[k]);  /* :172-173 */
+  }
+  /* boundary copies of ham (:68-80), then the axis / equator images (:87, :93) */
+  const int g = L->g, ni = L->ni, nj = L->nj;
+#define AT(i, j) (((i) + g - 1) + ((j) + g - 1) * ni)
+  for (int j = 1 - g; j <= L->Nz; ++j) L->ham[AT(1 - g, j)] = L->ham[AT(2 - g, j)];
+  for (int j = 1 - g; j <= L->Nz; ++j) L->ham[AT(L->Nr, j)] = L->ham[AT(L->Nr - 1, j)];
+  for (int i = 1 - g; i <= L->Nr; ++i) L->ham[AT(i, 1 - g)] = L->ham[AT(i, 2 - g)];
+  for (int i = 1 - g; i <= L->Nr; ++i) L->ham[AT(i, L->Nz)] = L->ham[AT(i, L->Nz - 1)];
+  for (int i = 1; i <= g; ++i)
+    for (int j = 1 - g; j <= L->Nz; ++j) L->ham[AT(1 - i, j)] = L->ham[AT(i, j)];
+  if (p->eqsym)
+    for (int j = 1; j <= g; ++j)
+      for (int i = 1 - g; i <= L->Nr; ++i) L->ham[AT(i, 1 - j)] = L->ham[AT(i, j)];
+#undef AT
+  (void)nj;
+}
+
 void or_sources_geometry(const oa_params* p, or_level* L) {
   const int shift = has_shift(p);
   const int matter = (p->mattertype != 0);

@@ -48,6 +48,7 @@
   X(Dr_scalar_xi_r) X(Dz_scalar_xi_r) X(DAr_scalar_xi_r) X(DAz_scalar_xi_r) \
   X(Dr_scalar_xi_z) X(Dz_scalar_xi_z) X(DAr_scalar_xi_z) X(DAz_scalar_xi_z) \
   X(scalar_V) X(scalar_VP) X(rho) X(J_r) X(J_z) X(S_A) X(S_B) X(S_H) X(S_C) X(trS) \
+  X(ham) X(mom_r) X(mom_z) X(CDelta_r) X(CDelta_z) \
   X(r) X(z) X(rr)
 
 typedef struct { const char* name; size_t off; } or_auxname;
@@ -500,6 +501,7 @@ double or_interp_at(or_handle* H, int lev, const double* f, double r0, double z0
 }
 
 int or_run_auxiliary(or_handle* H, int lev) { or_auxiliary(&H->p, &H->L[lev]); return 0; }
+int or_run_constraints(or_handle* H, int lev) { or_constraints(&H->p, &H->L[lev]); return 0; }
 int or_run_sources(or_handle* H, int lev) { or_level_sources(&H->p, &H->L[lev], lev); return 0; }
 int or_run_saveold(or_handle* H, int lev) { or_saveold(&H->p, &H->L[lev]); return 0; }
 int or_run_finebound(or_handle* H, int lev, double dtw) { or_finebound(H, lev, dtw); return 0; }

@@ -105,6 +105,8 @@ typedef struct {
   double *Dr_scalar_xi_z, *Dz_scalar_xi_z, *DAr_scalar_xi_z, *DAz_scalar_xi_z;
   double *scalar_V, *scalar_VP;
   double *rho, *J_r, *J_z, *S_A, *S_B, *S_H, *S_C, *trS;
+  /* constraints */
+  double *ham, *mom_r, *mom_z, *CDelta_r, *CDelta_z;
 } or_level;
 
 typedef struct or_handle {
@@ -144,6 +146,7 @@ void or_update(const oa_params* p, or_level* L, double dtw);
 
 void or_auxiliary(const oa_params* p, or_level* L);
 void or_auxiliary_geometry(const oa_params* p, or_level* L);
+void or_constraints(const oa_params* p, or_level* L);
 void or_auxiliary_matter(const oa_params* p, or_level* L);
 void or_potential(const oa_params* p, or_level* L);
 void or_stressenergy(const oa_params* p, or_level* L);

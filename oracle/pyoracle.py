@@ -49,6 +49,7 @@ def lib():
             "or_apply_dissipation": (ctypes.c_int, [H, ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
             "or_interp_at": (ctypes.c_double, [H, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double]),
             "or_run_auxiliary": (ctypes.c_int, [H, ctypes.c_int]),
+            "or_run_constraints": (ctypes.c_int, [H, ctypes.c_int]),
             "or_run_sources": (ctypes.c_int, [H, ctypes.c_int]),
             "or_run_saveold": (ctypes.c_int, [H, ctypes.c_int]),
             "or_run_finebound": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_double]),
@@ -113,6 +114,11 @@ class Oracle:
 
     def set(self, name, value, level=0):
         self.view(name, level)[...] = value
+
+    def constraints(self, level=0):
+        """constraints.f90 on a level from its last auxiliary arrays: fills
+        ham, mom_r, mom_z, CDelta_r, CDelta_z (read them with get())."""
+        lib().or_run_constraints(self._h, level)
 
     def initial_data(self):
         lib().or_initial_data(self._h)
