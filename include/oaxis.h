@@ -242,6 +242,13 @@ int oa_rhs_kernel_stats(const oa_handle* h, double* avg_ms, long* launches);
  * by this handle so far. */
 int oa_point_updates(const oa_handle* h, double* updates);
 
+/* constraints(box, level) (src/geometry/constraints.f90:36-206) on the
+ * level's current state: the Hamiltonian constraint ham, the momentum
+ * constraints mom_r, mom_z and the Delta constraints CDelta_r, CDelta_z,
+ * computed on the device from the state's geometric auxiliaries.  Read them
+ * with oa_download_field or oa_norms0d under those names, until the level's
+ * state changes. */
+int oa_constraints(oa_handle* h, int box, int level);
 /* 0D norms of an evolved array on one level, as save0Ddata.f90:221-288
  * forms them over i in [1, Nr-ghost], j in [1, Nz-ghost]: out[0] = max,
  * out[1] = min, out[2] = nm1 = max |f|, out[3] = nm2 = sqrt(sum f^2 / N).

@@ -150,6 +150,12 @@ int launch_deldef_table(const DelItem* items, int n, int blocks, void* stream);
 int launch_saveold_table(const SaveItem* items, int n, int blocks, void* stream);
 int launch_finebound_table(const FbItem* items, int n, int blocks, bool shift, bool scalar, void* stream);
 int launch_restrict_table(const RsItem* items, int n, int blocks, void* stream);
+// constraints of the state y (constraints.f90): the geometric auxiliaries
+// into gaux, then ham, mom_r, mom_z, CDelta_r, CDelta_z planes into out;
+// sc0: slot of scalar_phi when scalar
+int launch_constraints(const KGrid& G, const KPhys& P, const KStep& S, bool phimode, bool scalar, int sc0,
+                       const double* y, double* gaux, double* out, void* stream);
+
 // 0D norms (save0Ddata): norms_blocks() partial records of 6 doubles
 int norms_blocks();
 int launch_norms_partial(const KGrid& G, const double* f, double* part, void* stream);

@@ -29,6 +29,8 @@ struct Level {
   double t = 0.0, t1 = 0.0, t2 = 0.0;
   int s = 0;
   long strip_r = 0, strip_z = 0;                    // doubles per strip set piece
+  double* cons = nullptr;                           // ham, mom_r, mom_z, CDelta_r, CDelta_z (on demand)
+  bool cons_valid = false;                          // cons holds the constraints of the current state
   FbPoint* fbtab[2] = {nullptr, nullptr};           // fine-boundary geometry: depth g+2, depth g-1
   long fbn[2] = {0, 0};
 };

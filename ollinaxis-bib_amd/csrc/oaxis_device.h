@@ -985,6 +985,99 @@ OA_HD void saveold_point(const KGrid& G, const double* __restrict__ y, double* _
   }
 }
 
+// ------------------------------------------------------ constraints
+// constraints(box, level) (src/geometry/constraints.f90:36-206), hdet = 1,
+// no angmom: Hamiltonian, momentum and Delta constraints at (i, j), from the
+// geometric auxiliaries of the current state in gaux (RSCAL, DIV_KT) and
+// first derivatives read from the state planes.  Output planes, in order:
+// ham, mom_r, mom_z, CDelta_r, CDelta_z; the parity images are written with
+// the point.  The routine's smallpi = acos(-1.0) (:41) has a default-kind
+// (single precision) argument, so pi there is the single-precision value.
+enum ConsSlot { CN_ham = 0, CN_mom_r, CN_mom_z, CN_CDelta_r, CN_CDelta_z, NCONS };
+
+template <bool PHIMODE, bool SCALAR>
+OA_HD void constraints_point(const KGrid& G, const KPhys& P, int sc0, const double* __restrict__ y,
+                             const double* __restrict__ gaux, double* __restrict__ out, int i, int j) {
+  const long pl = G.plane;
+  const long k = IDX(G, i, j);
+  const GlobalView V{y, G, nullptr};
+  const double r = (static_cast<double>(i) - 0.5) * G.dr;
+  const double r2 = r * r;
+  const double ir = 1.0 / r;
+  const auto FA = V.fld(S_A), FB = V.fld(S_B), FH = V.fld(S_H), FC = V.fld(S_C);
+  const double chi = V.fld(S_chi)(i, j);
+  const double A = FA(i, j), B = FB(i, j), H = FH(i, j), C = FC(i, j);
+  const double trK = V.fld(S_trK)(i, j), KTA = V.fld(S_KTA)(i, j), KTB = V.fld(S_KTB)(i, j);
+  const double KTC = V.fld(S_KTC)(i, j);
+  // conformal factor derivatives, as the auxiliary call of this state formed them
+  double Drp, Dzp;
+  if constexpr (PHIMODE) {
+    const D1 dp = grad(V.fld(S_phi), G, i, j);
+    Drp = dp.r;
+    Dzp = dp.z;
+  } else {
+    const D1 dc = grad(V.fld(S_chi), G, i, j);
+    const double ic = 1.0 / chi;
+    Drp = -0.5 * (dc.r * ic);
+    Dzp = -0.5 * (dc.z * ic);
+  }
+  const double ipsi4 = chi * chi;
+  // conformal metric inverse (hdet = 1) and its derivatives
+  const double gA = B * H, gB = A * H, gH = 1.0 / H, gC = -C * H;
+  auto fgA = [&](int ii, int jj) { return FB(ii, jj) * FH(ii, jj); };
+  auto fgB = [&](int ii, int jj) { return FA(ii, jj) * FH(ii, jj); };
+  auto fgC = [&](int ii, int jj) { return -FC(ii, jj) * FH(ii, jj); };
+  const D1 dgA = grad(fgA, G, i, j), dgB = grad(fgB, G, i, j), dgC = grad(fgC, G, i, j);
+  const D1 dK = grad(V.fld(S_trK), G, i, j);
+  const double KTH = -(gA * KTA + gB * KTB + 2.0 * r2 * gC * KTC) * H;
+  const double lambda = (A - H) * ir * ir;
+  const double g_lambda = C * C - B * lambda;
+  const double r3 = r2 * r;
+  const double gA2 = gA * gA, gB2 = gB * gB, gC2 = gC * gC;
+  const double KTup_rr = (2.0 * gA * gC * KTC + gC2 * KTB) * r2 + gA2 * KTA;
+  const double KTup_zz = (gC2 * KTA + 2.0 * gC * gB * KTC) * r2 + gB2 * KTB;
+  const double KTup_rz = gC2 * r3 * KTC + (gC * gA * KTA + gB * gA * KTC + gB * gC * KTB) * r;
+  const double KT2_A = (2.0 * KTA * gC * KTC + KTC * KTC * gB) * r2 + gA * KTA * KTA;
+  const double KT2_B = (KTC * KTC * gA + 2.0 * KTC * gC * KTB) * r2 + gB * KTB * KTB;
+  const double KT2_H = KTH * KTH * gH;
+  const double KT2_C = KTC * KTC * gC * r2 + KTA * gA * KTC + KTA * gC * KTB + KTC * gB * KTB;
+  const double KT2 = gA * KT2_A + gB * KT2_B + gH * KT2_H + 2.0 * gC * KT2_C * r2;
+  const double third = 1.0 / 3.0, two = 2.0, half = 0.5;
+  const double* const ax = gaux + k;
+  const double RSCAL = ax[AUX_RSCAL * pl], DIV_r = ax[AUX_DIV_r * pl], DIV_z = ax[AUX_DIV_z * pl];
+  double ham = RSCAL - KT2 + two * third * trK * trK;                                  // :57
+  double mom_r = DIV_r + 6.0 * (KTup_rr * Drp + KTup_rz * Dzp) - two * third * (gA * dK.r + r * gC * dK.z);
+  double mom_z = DIV_z + 6.0 * (KTup_rz * Drp + KTup_zz * Dzp) - two * third * (r * gC * dK.r + gB * dK.z);
+  if constexpr (SCALAR) {
+    const double smallpi = static_cast<double>(3.14159274101257324f);   // acos(-1.0) in single precision
+    const double sphi = V.fld(sc0 + SC_phi)(i, j), xr = V.fld(sc0 + SC_xi_r)(i, j);
+    const double xz = V.fld(sc0 + SC_xi_z)(i, j), pi = V.fld(sc0 + SC_pi)(i, j);
+    double Vp = 0.0;
+    if (P.scalarpotential == OA_POT_PHI2) {
+      Vp = 0.5 * P.scalar_mass * P.scalar_mass * sphi * sphi;
+    } else if (P.scalarpotential == OA_POT_PHI4) {
+      Vp = 0.5 * P.scalar_mass * P.scalar_mass * sphi * sphi + 0.25 * P.scalar_lambda * sphi * sphi * sphi * sphi;
+    }
+    const double q2 = (gA * xr * xr + gB * xz * xz + 2.0 * r * gC * xr * xz) * ipsi4;
+    const double rho = 0.5 * (pi * pi + q2) + Vp;
+    const double J_r = -(gA * xr + r * gC * xz) * pi * ipsi4;
+    const double J_z = -(gB * xz + r * gC * xr) * pi * ipsi4;
+    const double psi4 = 1.0 / ipsi4;
+    ham = ham - 16.0 * smallpi * rho;                                                   // :60
+    mom_r = mom_r - 8.0 * smallpi * J_r * psi4;                                         // :120
+    mom_z = mom_z - 8.0 * smallpi * J_z * psi4;                                         // :133
+  }
+  // hdet = 1: the Dr_hdet, Dz_hdet terms vanish
+  const double CDr = V.fld(S_Delta_r)(i, j) + (dgA.r + r * dgC.z + r * g_lambda);     // :165-166
+  const double CDz = V.fld(S_Delta_z)(i, j) + (dgC.r * r + dgB.z + two * gC);          // :172-173
+  (void)half;
+  put(out + CN_ham * pl, G, i, j, ham, 1.0, 1.0);
+  put(out + CN_mom_r * pl, G, i, j, mom_r, -1.0, 1.0);
+  put(out + CN_mom_z * pl, G, i, j, mom_z, 1.0, -1.0);
+  put(out + CN_CDelta_r * pl, G, i, j, CDr, -1.0, 1.0);
+  put(out + CN_CDelta_z * pl, G, i, j, CDz, 1.0, -1.0);
+}
+
 // ------------------------------------------------------ interpolation
 // interp.f90:140-263 on a level held in z-fastest planes: the cell search on
 // floating coordinates, the exact-node copy, the bilinear fallback near edges

@@ -66,6 +66,14 @@ int slot_of(const oa_handle* h, const std::string& name) {
   return -1;
 }
 
+// constraint array names (arrays.config) and their planes in Level::cons
+int cons_slot(const std::string& name) {
+  static const char* names[] = {"ham", "mom_r", "mom_z", "CDelta_r", "CDelta_z"};
+  for (int c = 0; c < 5; ++c)
+    if (name == names[c]) return c;
+  return -1;
+}
+
 void fill_phys(oa_handle* h) {
   const oa_params& p = h->p;
   KPhys& P = h->phys;
@@ -183,6 +191,7 @@ int prepare_substep(oa_handle* h, int level, int iter, int niter, double dtw, do
   it->gaux = L.gaux;
   L.aux_phi = time0;
   L.dtb_pre_valid = false;
+  L.cons_valid = false;
   L.last_out = out;
   if (!time0 && h->p.chimethod) L.phi_valid = false;
   if (iter == niter) {
@@ -261,6 +270,7 @@ void prepare_restrict(oa_handle* h, int level, RsItem* it) {
   it->yf = L.slab[L.cur];
   it->yc = Lc.slab[Lc.cur];
   Lc.dirty = true;
+  Lc.cons_valid = false;
 }
 
 void finish_level_step(oa_handle* h, int level) {
@@ -450,6 +460,7 @@ extern "C" int oa_destroy(oa_handle* h) {
     if (L.gaux) (void)hipFree(L.gaux);
     for (int k = 0; k < 2; ++k)
       if (L.fbtab[k]) (void)hipFree(L.fbtab[k]);
+    if (L.cons) (void)hipFree(L.cons);
   }
   h->plan.release();
   if (h->norms_part) (void)hipFree(h->norms_part);
@@ -500,6 +511,7 @@ extern "C" int oa_upload_field(oa_handle* h, int box, int level, const char* nam
                         sizeof(double) * plane.size(), hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   L->dirty = true;
+  L->cons_valid = false;
   if (slot == S_phi) L->phi_valid = true;
   return OA_OK;
 }
@@ -561,6 +573,18 @@ extern "C" int oa_download_field(oa_handle* h, int box, int level, const char* n
     });
     return OA_OK;
   }
+  if (cons_slot(nm) >= 0) {
+    if (!L->cons_valid) {
+      oa_set_error(std::string("oa_download_field: ") + name + " needs oa_constraints on the current state first");
+      return OA_ERR_STATE;
+    }
+    std::vector<double> b(static_cast<size_t>(G.plane));
+    HIPCHK(hipMemcpyAsync(b.data(), L->cons + static_cast<long>(cons_slot(nm)) * G.plane, sizeof(double) * b.size(),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    emit([&](size_t q, int, int) { return b[q]; });
+    return OA_OK;
+  }
   if (nm == "dtalpha" || nm == "dtphi" || nm == "z4theta") {
     emit([&](size_t, int, int) { return 0.0; });
     return OA_OK;
@@ -569,16 +593,42 @@ extern "C" int oa_download_field(oa_handle* h, int box, int level, const char* n
   return OA_ERR_ARG;
 }
 
+extern "C" int oa_constraints(oa_handle* h, int box, int level) {
+  Level* L = level_of(h, box, level);
+  if (!L) return OA_ERR_ARG;
+  if (L->dirty) {
+    oa_set_error("oa_constraints: state not projected (call oa_auxiliary / oa_initial_aux)");
+    return OA_ERR_STATE;
+  }
+  if (!L->cons) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMalloc(reinterpret_cast<void**>(&L->cons), sizeof(double) * 5 * static_cast<size_t>(L->G.plane)));
+    HIPCHK(hipMemsetAsync(L->cons, 0, sizeof(double) * 5 * static_cast<size_t>(L->G.plane), h->stream));
+  }
+  KStep S{1, 1, 0, 0.0, 0.0, L->t == 0.0 ? 1 : 0, nullptr};
+  const double* y = L->slab[L->in_step ? L->last_out : L->cur];
+  RC(launch_constraints(L->G, h->phys, S, L->aux_phi, h->scalar, scalar_slot(h->shift, SC_phi), y, L->gaux,
+                        L->cons, h->stream));
+  L->cons_valid = true;
+  return OA_OK;
+}
+
 extern "C" int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out) {
   Level* L = level_of(h, box, level);
   if (!L || !name || !out) return OA_ERR_ARG;
   const int slot = slot_of(h, name);
-  if (slot < 0 || (slot == S_phi && !L->phi_valid)) {
-    oa_set_error(std::string("oa_norms0d: ") + name + " is not a stored evolved array of this configuration");
+  const int cslot = cons_slot(name);
+  const double* plane = nullptr;
+  if (slot >= 0 && !(slot == S_phi && !L->phi_valid)) {
+    plane = L->slab[L->in_step ? L->last_out : L->cur] + static_cast<long>(slot) * L->G.plane;
+  } else if (cslot >= 0 && L->cons_valid) {
+    plane = L->cons + static_cast<long>(cslot) * L->G.plane;
+  } else {
+    oa_set_error(std::string("oa_norms0d: ") + name +
+                 " is not a stored evolved array of this configuration (constraints need oa_constraints first)");
     return OA_ERR_ARG;
   }
-  const double* y = L->slab[L->in_step ? L->last_out : L->cur];
-  RC(launch_norms_partial(L->G, y + static_cast<long>(slot) * L->G.plane, h->norms_part, h->stream));
+  RC(launch_norms_partial(L->G, plane, h->norms_part, h->stream));
   const int nb = norms_blocks();
   std::vector<double> part(static_cast<size_t>(6 * nb));
   HIPCHK(hipMemcpyAsync(part.data(), h->norms_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));

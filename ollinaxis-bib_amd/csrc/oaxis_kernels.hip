@@ -159,6 +159,69 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_rhs_table(const RhsItem* _
   rhs_tile<SHIFT, SCALAR, PHIMODE>(it, t / it.nbj, t % it.nbj, tile);
 }
 
+// ------------------------------------------------------ constraints
+// The geometric auxiliaries of the current state (phase 1 of the RHS kernel
+// alone, on the vacuum fields), for the constraints.
+template <bool PHIMODE>
+__global__ void __launch_bounds__(kTile * kTile, 2) k_aux(RhsItem it) {
+  extern __shared__ double tile[];
+  const KGrid& G = it.G;
+  const int bi = static_cast<int>(blockIdx.y), bj = static_cast<int>(blockIdx.x);
+  const int j0 = tile_start(bj, it.nbj, G.jlo, G.Nz);
+  const int i0 = tile_start(bi, it.nbi, 1, G.Nr);
+  const int ib = i0 - kHalo, jb = j0 - kHalo;
+  const double* __restrict__ yin = it.yin;
+  stage_tile(G, tile, NBASE, [&](int f) { return yin + f * G.plane; }, ib, jb);
+  const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
+  if (bi == it.nbi - 1 && i < 1 + (it.nbi - 1) * kTile) return;     // done by the tile before
+  if (bj == it.nbj - 1 && j < G.jlo + (it.nbj - 1) * kTile) return;
+  const TileView V{tile, ib, jb, kTileW, kTileSize, NBASE};
+  if (interior_tile(G, i0, j0))
+    rhs_point<false, false, PHIMODE, true, 1>(G, it.P, it.S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
+  else
+    rhs_point<false, false, PHIMODE, false, 1>(G, it.P, it.S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
+}
+
+struct ConsArgs {
+  KGrid G; KPhys P; int sc0;
+  const double* y; const double* gaux; double* out;
+};
+
+template <bool PHIMODE, bool SCALAR>
+__global__ void __launch_bounds__(256) k_constraints(ConsArgs c) {
+  const int j = c.G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > c.G.Nz || i > c.G.Nr) return;
+  constraints_point<PHIMODE, SCALAR>(c.G, c.P, c.sc0, c.y, c.gaux, c.out, i, j);
+}
+
+// constraints.f90:68-93, in the reference's order: ham copied from the
+// neighbouring row at i = 1-ghost, i = Nr, j = 1-ghost, j = Nz, then the
+// axis images and (equatorial symmetry) the equator images.  One block;
+// each step sees the previous one's writes.
+__global__ void __launch_bounds__(256) k_ham_fix(KGrid G, double* __restrict__ ham) {
+  const int t = static_cast<int>(threadIdx.x), nt = static_cast<int>(blockDim.x);
+  const int g = G.g;
+  for (int j = 1 - g + t; j <= G.Nz; j += nt) ham[IDX(G, 1 - g, j)] = ham[IDX(G, 2 - g, j)];
+  __syncthreads();
+  for (int j = 1 - g + t; j <= G.Nz; j += nt) ham[IDX(G, G.Nr, j)] = ham[IDX(G, G.Nr - 1, j)];
+  __syncthreads();
+  for (int i = 1 - g + t; i <= G.Nr; i += nt) ham[IDX(G, i, 1 - g)] = ham[IDX(G, i, 2 - g)];
+  __syncthreads();
+  for (int i = 1 - g + t; i <= G.Nr; i += nt) ham[IDX(G, i, G.Nz)] = ham[IDX(G, i, G.Nz - 1)];
+  __syncthreads();
+  for (int q = t; q < g * (G.Nz + g); q += nt) {
+    const int i = 1 + q / (G.Nz + g), j = 1 - g + q % (G.Nz + g);
+    ham[IDX(G, 1 - i, j)] = ham[IDX(G, i, j)];
+  }
+  __syncthreads();
+  if (G.eqsym)
+    for (int q = t; q < g * (G.Nr + g); q += nt) {
+      const int j = 1 + q / (G.Nr + g), i = 1 - g + q % (G.Nr + g);
+      ham[IDX(G, i, 1 - j)] = ham[IDX(G, i, j)];
+    }
+}
+
 // ----------------------------------------------- pointwise kernels
 // One thread per point (or record) of an item, 256 threads per block; q is
 // the thread's index within its item.
@@ -468,6 +531,41 @@ int norms_blocks() { return kNormBlocks; }
 int launch_norms_partial(const KGrid& G, const double* f, double* part, void* stream) {
   hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), G, f,
                      part);
+  return check_launch();
+}
+
+int launch_constraints(const KGrid& G, const KPhys& P, const KStep& S, bool phimode, bool scalar, int sc0,
+                       const double* y, double* gaux, double* out, void* stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  RhsItem it{};
+  RC_K(rhs_tiles(G, &it.nbi, &it.nbj));
+  it.G = G;
+  it.P = P;
+  it.S = S;
+  it.S.dtb_pre = nullptr;
+  it.yin = y;
+  it.gaux = gaux;
+  const size_t lds = sizeof(double) * kTileSize * NBASE;
+  static bool attr = false;
+  if (!attr) {
+    RC_K(raise_lds_limit(&k_aux<false>, lds));
+    RC_K(raise_lds_limit(&k_aux<true>, lds));
+    attr = true;
+  }
+  const dim3 tg(static_cast<unsigned>(it.nbj), static_cast<unsigned>(it.nbi), 1);
+  if (phimode) hipLaunchKernelGGL(k_aux<true>, tg, dim3(kTile, kTile, 1), lds, st, it);
+  else hipLaunchKernelGGL(k_aux<false>, tg, dim3(kTile, kTile, 1), lds, st, it);
+  RC_K(check_launch());
+  const ConsArgs c{G, P, sc0, y, gaux, out};
+  const dim3 blk(64, 4, 1);
+  const dim3 grd((G.Nz - G.jlo + 1 + 63) / 64, (G.Nr + 3) / 4, 1);
+  const int code = (phimode ? 2 : 0) + (scalar ? 1 : 0);
+  if (code == 0) hipLaunchKernelGGL((k_constraints<false, false>), grd, blk, 0, st, c);
+  else if (code == 1) hipLaunchKernelGGL((k_constraints<false, true>), grd, blk, 0, st, c);
+  else if (code == 2) hipLaunchKernelGGL((k_constraints<true, false>), grd, blk, 0, st, c);
+  else hipLaunchKernelGGL((k_constraints<true, true>), grd, blk, 0, st, c);
+  RC_K(check_launch());
+  hipLaunchKernelGGL(k_ham_fix, dim3(1), dim3(256), 0, st, G, out + CN_ham * G.plane);
   return check_launch();
 }
 

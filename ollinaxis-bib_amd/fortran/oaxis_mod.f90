@@ -26,7 +26,8 @@ module oaxis_mod
             oa_batch_create, oa_batch_destroy, oa_batch_add, oa_batch_remove, oa_batch_size, &
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
-            oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule
+            oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule, &
+            oa_constraints
 
   ! storage for one oa_params block
   type :: oaxis_params
@@ -229,6 +230,12 @@ module oaxis_mod
       type(c_ptr), value :: h
       real(c_double) :: avg_ms
       integer(c_long) :: launches
+    end function
+    ! constraints.f90 on a level's current state (ham, mom_r, mom_z, CDelta_r, CDelta_z)
+    integer(c_int) function oa_constraints(h, box, level) bind(C, name="oa_constraints")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level
     end function
     ! 0D norms (max, min, nm1, nm2) of an evolved array, as save0Ddata forms them
     integer(c_int) function oa_norms0d(h, box, level, name, norms) bind(C, name="oa_norms0d")

@@ -191,6 +191,11 @@ class Spacetime:
     def advance_time(self, level):
         _check(lib().oa_advance_time(self._h, 0, level), "oa_advance_time")
 
+    def constraints(self, level=0):
+        """constraints.f90 on a level's current state: afterwards ham, mom_r,
+        mom_z, CDelta_r, CDelta_z can be downloaded or reduced by name."""
+        _check(lib().oa_constraints(self._h, 0, level), "oa_constraints")
+
     def norms0d(self, name, level=0):
         """save0Ddata's norms of an evolved array: dict(max, min, nm1, nm2)."""
         out = (ctypes.c_double * 4)()

@@ -108,6 +108,7 @@ def load_library(path=LIBPATH):
         "oa_rhs_kernel_stats": (ctypes.c_int, [H, dp, ctypes.POINTER(ctypes.c_long)]),
         "oa_point_updates": (ctypes.c_int, [H, dp]),
         "oa_norms0d": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp]),
+        "oa_constraints": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
         "oa_params_footprint": (ctypes.c_int, [P, dp]),
         "oa_batch_create": (ctypes.c_int, [ctypes.POINTER(H)]),
         "oa_batch_destroy": (ctypes.c_int, [H]),
