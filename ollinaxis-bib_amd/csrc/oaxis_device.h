@@ -437,163 +437,173 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
     RA = ax[AUX_RA * pl]; RB = ax[AUX_RB * pl]; RC = ax[AUX_RC * pl]; RSCAL = ax[AUX_RSCAL * pl];
     DIV_r = ax[AUX_DIV_r * pl]; DIV_z = ax[AUX_DIV_z * pl]; DIVA_r = ax[AUX_DIVA_r * pl]; DIVA_z = ax[AUX_DIVA_z * pl];
   } else {
-    // ---- D_i D_j alpha and its trace (:1268-1324)
-    d2A = da.rr - Dra * gC * r2 * dC.r
-                 - (Dza * gB * dC.r + 0.5 * Dza * gC * dA.r + Dra * gC * C - 0.5 * Dra * gC * dA.z) * r
-                 - 0.5 * Dra * gA * dA.r - Dza * gB * C + 0.5 * Dza * gB * dA.z;
-    d2B = da.zz - Dza * gC * r2 * dC.z
-                 - (Dra * gA * dC.z + 0.5 * Dra * gC * dB.z - 0.5 * Dza * gC * dB.r) * r
-                 + 0.5 * Dra * gA * dB.r - 0.5 * Dza * gB * dB.z;
-    double d2H = (0.5 * Dra * gC * dH.z + 0.5 * Dza * gC * dH.r) * r
-                 + 0.5 * Dra * gA * dH.r + Dza * gC * H + 0.5 * Dza * gB * dH.z + Dra * gA * H * ir;
-    d2C = da.rz * ir - 0.5 * Dra * gC * dB.r - 0.5 * Dza * gC * dA.z
-                 - (0.5 * Dra * gA * dA.z + 0.5 * Dza * gB * dB.r) * ir;
-    {
-      const double aux = gA * Dra * Drp + gB * Dza * Dzp + r * gC * (Dra * Dzp + Dza * Drp);
-      d2A -= 2.0 * (2.0 * Dra * Drp - A * aux);
-      d2B -= 2.0 * (2.0 * Dza * Dzp - B * aux);
-      d2H += 2.0 * H * aux;
-      d2C -= 2.0 * (Dra * ir * Dzp + Dza * Drp * ir - C * aux);
-    }
-    Lapla = (gA * d2A + gB * d2B + gH * d2H + 2.0 * r2 * gC * d2C) * ipsi4;
+    if constexpr (STAGE != 12) {
+      // ---- D_i D_j alpha and its trace (:1268-1324)
+      d2A = da.rr - Dra * gC * r2 * dC.r
+                   - (Dza * gB * dC.r + 0.5 * Dza * gC * dA.r + Dra * gC * C - 0.5 * Dra * gC * dA.z) * r
+                   - 0.5 * Dra * gA * dA.r - Dza * gB * C + 0.5 * Dza * gB * dA.z;
+      d2B = da.zz - Dza * gC * r2 * dC.z
+                   - (Dra * gA * dC.z + 0.5 * Dra * gC * dB.z - 0.5 * Dza * gC * dB.r) * r
+                   + 0.5 * Dra * gA * dB.r - 0.5 * Dza * gB * dB.z;
+      double d2H = (0.5 * Dra * gC * dH.z + 0.5 * Dza * gC * dH.r) * r
+                   + 0.5 * Dra * gA * dH.r + Dza * gC * H + 0.5 * Dza * gB * dH.z + Dra * gA * H * ir;
+      d2C = da.rz * ir - 0.5 * Dra * gC * dB.r - 0.5 * Dza * gC * dA.z
+                   - (0.5 * Dra * gA * dA.z + 0.5 * Dza * gB * dB.r) * ir;
+      {
+        const double aux = gA * Dra * Drp + gB * Dza * Dzp + r * gC * (Dra * Dzp + Dza * Drp);
+        d2A -= 2.0 * (2.0 * Dra * Drp - A * aux);
+        d2B -= 2.0 * (2.0 * Dza * Dzp - B * aux);
+        d2H += 2.0 * H * aux;
+        d2C -= 2.0 * (Dra * ir * Dzp + Dza * Drp * ir - C * aux);
+      }
+      Lapla = (gA * d2A + gB * d2B + gH * d2H + 2.0 * r2 * gC * d2C) * ipsi4;
 
-    // ---- Ricci tensor (:1397-1493 and calc_conformalRicci1..4, non-angmom)
-    double RH;
-    {
-      RA = -2.0 * (dp.rr - Drp * gC * r2 * dC.r
-                   - (Dzp * gB * dC.r + 0.5 * Dzp * gC * dA.r + Drp * gC * C - 0.5 * Drp * gC * dA.z) * r
-                   - 0.5 * Drp * gA * dA.r - Dzp * gB * C + 0.5 * Dzp * gB * dA.z);
-      RB = -2.0 * (dp.zz - Dzp * gC * r2 * dC.z
-                   - (Drp * gA * dC.z + 0.5 * Drp * gC * dB.z - 0.5 * Dzp * gC * dB.r) * r
-                   + 0.5 * Drp * gA * dB.r - 0.5 * Dzp * gB * dB.z);
-      RH = -2.0 * ((0.5 * Drp * gC * dH.z + 0.5 * Dzp * gC * dH.r) * r
-                   + 0.5 * Drp * gA * dH.r + Dzp * gC * H + 0.5 * Dzp * gB * dH.z + Drp * gA * H * ir);
-      RC = -2.0 * (dp.rz * ir - 0.5 * Drp * gC * dB.r - 0.5 * Dzp * gC * dA.z
-                   - (0.5 * Drp * gA * dA.z + 0.5 * Dzp * gB * dB.r) * ir);
-      double aux = gA * RA + gB * RB + gH * RH + 2.0 * r2 * gC * RC;
-      RA += A * aux; RB += B * aux; RH += H * aux; RC += C * aux;
-      RA += 4.0 * Drp * Drp;
-      RB += 4.0 * Dzp * Dzp;
-      RC += 4.0 * Drp * Dzp * ir;
-      aux = -2.0 * (gA * Drp * Drp + gB * Dzp * Dzp + 2.0 * r * gC * Drp * Dzp);
-      RA += 2.0 * A * aux; RB += 2.0 * B * aux; RH += 2.0 * H * aux; RC += 2.0 * C * aux;
-    }
-    {
-      // term 1: second derivatives of the conformal metric
-      const double lamDef = lambda;
-      RA += -gC * dA.rz * r - 0.5 * gB * dA.zz - 0.5 * gA * dA.rr - 0.5 * gH * dA.r * ir + gH * lamDef;
-      RB += -gC * dB.rz * r - 0.5 * gA * dB.rr - 0.5 * gB * dB.zz - 0.5 * gH * dB.r * ir;
-      RH += -gC * dH.rz * r - 0.5 * gB * dH.zz - 0.5 * gA * dH.rr - 0.5 * gH * dH.r * ir - gH * lamDef;
-      RC += -gC * dC.rz * r - 0.5 * gA * dC.rr - gC * dC.z - 0.5 * gB * dC.zz
-            + (-gA * dC.r - 0.5 * gH * dC.r) * ir;
-      // term 2: Delta terms
-      RA += C * dDz.r * r + A * dDr.r + 0.5 * (DelDef_r * dA.r + DelDef_z * dA.z);
-      RB += C * dDr.z * r + B * dDz.z + 0.5 * (DelDef_r * dB.r + DelDef_z * dB.z);
-      RH += 0.5 * (DelDef_r * dH.r + DelDef_z * dH.z) + Delta_r * H * ir;
-      RC += 0.5 * C * (dDr.r + dDz.z) + 0.5 * (DelDef_r * dC.r + DelDef_z * dC.z)
-            + 0.5 * (C * Delta_r + A * dDr.z + B * dDz.r) * ir;
-    }
-    {
-      // terms 3 and 4: quadratic in first derivatives of the metric
-      const double Ar = dA.r, Az = dA.z, Br = dB.r, Bz = dB.z, Hr = dH.r, Hz = dH.z, Cr = dC.r, Cz = dC.z;
-      RA += (2.0 * gB * gC * Cr * Cz + gC2 * Ar * Cz + Az * gC2 * Cr) * r3
-            + (2.0 * gB * gC * C * Cz - gB * gC * Cr * Br + Ar * gC * gA * Cr + Az * gC2 * C) * r2
-            + (Az * gB * gC * Br + Az * gB * gA * Cr + Ar * gC * gA * C - gB * gC * C * Br + Az * gC * gA * Ar
-               + gB2 * Br * Cz) * r
-            - 0.5 * gH2 * Hr * Hr - 0.5 * gB2 * Br * Br + gB * gA * C * Az + 0.5 * Ar * Ar * gA2
-            + (gH * Hr + gH2 * A * Hr - 2.0 * gH2 * Hr * H) * ir;
-      RB += (Br * gC2 * Cz + gC2 * Bz * Cr + 2.0 * gA * gC * Cz * Cr) * r3
-            + (2.0 * gA * gC * Cz * C - gA * gC * Cz * Az + gC2 * Bz * C + Bz * gC * gB * Cz) * r2
-            + (Br * gA * gC * Az + gA2 * Az * Cr + Br * gA * gB * Cz + Br * gB * gC * Bz) * r
-            + 0.5 * gB2 * Bz * Bz + gA2 * Az * C - 0.5 * gA2 * Az * Az + gH2 * Hz * C - 0.5 * gH2 * Hz * Hz;
-      RH += -gC * Hz - r * g_lambda * Hr;
-      RC += (gC2 * Br * Cr + gC * gB * Cz * Cz + Az * gC2 * Cz + gC * gA * Cr * Cr) * r2
-            + (-0.5 * gC * gA * Cr * Az + 2.0 * gC * gA * Cr * C - 0.5 * gC2 * Br * Az - 0.5 * gC * gB * Cz * Br
-               + 0.5 * Bz * gB * gC * Cr + gC2 * Br * C + 0.5 * Bz * gC2 * Ar + 0.5 * Ar * gA * gC * Cz) * r
-            + 0.5 * Br * gC * gA * Ar + 0.5 * Br * gB * gA * Cr + 0.5 * gB2 * Bz * Cz + 0.5 * Az * gA * gB * Cz
-            + 0.5 * gA2 * Ar * Cr + 0.5 * Br * Br * gB * gC + gC * gA * C * C - 0.5 * gC * gA * C * Az
-            + 0.5 * Az * gB * gC * Bz + 0.5 * Az * Az * gA * gC + 0.5 * Bz * gB * gC * C
-            + (0.5 * gH2 * C * Hr + 0.5 * gB * gA * Br * C - 0.5 * gH2 * Hz * Hr + 0.5 * gA2 * Ar * C) * ir
-            + 0.5 * Hz * gH2 * lambda;
-      RA += Az * gC2 * Cr * r3
-            + (-0.5 * Az * Az * gC2 + Az * gC2 * C + gB * gA * Cr * Cr + 0.5 * Ar * gC2 * Br + Ar * gC * gA * Cr
-               + gB * gC * Cr * Br) * r2
-            + (-Az * gB * gA * Cr + 2.0 * gB * gA * Cr * C + gB * gC * C * Br + Ar * gC * gA * C) * r
-            + 0.25 * gH2 * Hr * Hr - gB * gA * C * Az + 0.25 * gB2 * Br * Br + gB * gA * C * C
-            + 0.25 * Ar * Ar * gA2 + 0.5 * gB * gA * Az * Az;
-      RB += Br * gC2 * r3 * Cz
-            + (gA * gB * Cz * Cz + 0.5 * Az * gC2 * Bz - 0.5 * gC2 * Br * Br + Bz * gC * gB * Cz + gA * gC * Cz * Az) * r2
-            - Br * gA * gB * Cz * r + 0.25 * gA2 * Az * Az + 0.5 * gB * gA * Br * Br + 0.25 * gH2 * Hz * Hz
-            + 0.25 * gB2 * Bz * Bz;
-      RH += (-0.5 * C * gC * gH * Hr + gH * gC * Hz * Hr) * r + 0.5 * gH * Hz * Hz * gB + 1.5 * gC * Hz
-            + 0.5 * gH * gA * Hr * Hr - 0.5 * C * gB * gH * Hz - 0.5 * gH * A * gC * Hz - C * gC - 0.5 * gC * Hz
-            + 0.5 * (2.0 * gA - gH - gH * A * gA) * Hr * ir + gH * lambda * (1.0 - gA * H);
-      RC += gC2 * Cr * Cz * r3 + (-0.5 * Az * gC2 * Cz + gC2 * C * Cz - 0.5 * gC2 * Br * Cr) * r2
-            + (-0.5 * gC2 * Br * C + 0.25 * Bz * gC2 * Ar + 0.5 * Ar * gA * gC * Cz + 0.5 * gC * gB * Cz * Br
-               + 0.75 * gC2 * Br * Az + 0.5 * Bz * gB * gC * Cr + 0.5 * gC * gA * Cr * Az) * r
-            + 0.5 * Az * gA * gB * Cz + 0.5 * Bz * gB * gC * C + 0.5 * gC * gA * C * Az + 0.5 * Br * gB * gA * Cr
-            + (0.25 * gH2 * Hz * Hr + 0.5 * gB * gA * Br * C + 0.25 * gA2 * Ar * Az + 0.25 * Bz * gB2 * Br
-               - 0.5 * gB * gA * Az * Br) * ir;
-    }
-    RSCAL = (gA * RA + gB * RB + gH * RH + 2.0 * r2 * gC * RC) * ipsi4;
+      // ---- Ricci tensor (:1397-1493 and calc_conformalRicci1..4, non-angmom)
+      double RH;
+      {
+        RA = -2.0 * (dp.rr - Drp * gC * r2 * dC.r
+                     - (Dzp * gB * dC.r + 0.5 * Dzp * gC * dA.r + Drp * gC * C - 0.5 * Drp * gC * dA.z) * r
+                     - 0.5 * Drp * gA * dA.r - Dzp * gB * C + 0.5 * Dzp * gB * dA.z);
+        RB = -2.0 * (dp.zz - Dzp * gC * r2 * dC.z
+                     - (Drp * gA * dC.z + 0.5 * Drp * gC * dB.z - 0.5 * Dzp * gC * dB.r) * r
+                     + 0.5 * Drp * gA * dB.r - 0.5 * Dzp * gB * dB.z);
+        RH = -2.0 * ((0.5 * Drp * gC * dH.z + 0.5 * Dzp * gC * dH.r) * r
+                     + 0.5 * Drp * gA * dH.r + Dzp * gC * H + 0.5 * Dzp * gB * dH.z + Drp * gA * H * ir);
+        RC = -2.0 * (dp.rz * ir - 0.5 * Drp * gC * dB.r - 0.5 * Dzp * gC * dA.z
+                     - (0.5 * Drp * gA * dA.z + 0.5 * Dzp * gB * dB.r) * ir);
+        double aux = gA * RA + gB * RB + gH * RH + 2.0 * r2 * gC * RC;
+        RA += A * aux; RB += B * aux; RH += H * aux; RC += C * aux;
+        RA += 4.0 * Drp * Drp;
+        RB += 4.0 * Dzp * Dzp;
+        RC += 4.0 * Drp * Dzp * ir;
+        aux = -2.0 * (gA * Drp * Drp + gB * Dzp * Dzp + 2.0 * r * gC * Drp * Dzp);
+        RA += 2.0 * A * aux; RB += 2.0 * B * aux; RH += 2.0 * H * aux; RC += 2.0 * C * aux;
+      }
+      {
+        // term 1: second derivatives of the conformal metric
+        const double lamDef = lambda;
+        RA += -gC * dA.rz * r - 0.5 * gB * dA.zz - 0.5 * gA * dA.rr - 0.5 * gH * dA.r * ir + gH * lamDef;
+        RB += -gC * dB.rz * r - 0.5 * gA * dB.rr - 0.5 * gB * dB.zz - 0.5 * gH * dB.r * ir;
+        RH += -gC * dH.rz * r - 0.5 * gB * dH.zz - 0.5 * gA * dH.rr - 0.5 * gH * dH.r * ir - gH * lamDef;
+        RC += -gC * dC.rz * r - 0.5 * gA * dC.rr - gC * dC.z - 0.5 * gB * dC.zz
+              + (-gA * dC.r - 0.5 * gH * dC.r) * ir;
+        // term 2: Delta terms
+        RA += C * dDz.r * r + A * dDr.r + 0.5 * (DelDef_r * dA.r + DelDef_z * dA.z);
+        RB += C * dDr.z * r + B * dDz.z + 0.5 * (DelDef_r * dB.r + DelDef_z * dB.z);
+        RH += 0.5 * (DelDef_r * dH.r + DelDef_z * dH.z) + Delta_r * H * ir;
+        RC += 0.5 * C * (dDr.r + dDz.z) + 0.5 * (DelDef_r * dC.r + DelDef_z * dC.z)
+              + 0.5 * (C * Delta_r + A * dDr.z + B * dDz.r) * ir;
+      }
+      {
+        // terms 3 and 4: quadratic in first derivatives of the metric
+        const double Ar = dA.r, Az = dA.z, Br = dB.r, Bz = dB.z, Hr = dH.r, Hz = dH.z, Cr = dC.r, Cz = dC.z;
+        RA += (2.0 * gB * gC * Cr * Cz + gC2 * Ar * Cz + Az * gC2 * Cr) * r3
+              + (2.0 * gB * gC * C * Cz - gB * gC * Cr * Br + Ar * gC * gA * Cr + Az * gC2 * C) * r2
+              + (Az * gB * gC * Br + Az * gB * gA * Cr + Ar * gC * gA * C - gB * gC * C * Br + Az * gC * gA * Ar
+                 + gB2 * Br * Cz) * r
+              - 0.5 * gH2 * Hr * Hr - 0.5 * gB2 * Br * Br + gB * gA * C * Az + 0.5 * Ar * Ar * gA2
+              + (gH * Hr + gH2 * A * Hr - 2.0 * gH2 * Hr * H) * ir;
+        RB += (Br * gC2 * Cz + gC2 * Bz * Cr + 2.0 * gA * gC * Cz * Cr) * r3
+              + (2.0 * gA * gC * Cz * C - gA * gC * Cz * Az + gC2 * Bz * C + Bz * gC * gB * Cz) * r2
+              + (Br * gA * gC * Az + gA2 * Az * Cr + Br * gA * gB * Cz + Br * gB * gC * Bz) * r
+              + 0.5 * gB2 * Bz * Bz + gA2 * Az * C - 0.5 * gA2 * Az * Az + gH2 * Hz * C - 0.5 * gH2 * Hz * Hz;
+        RH += -gC * Hz - r * g_lambda * Hr;
+        RC += (gC2 * Br * Cr + gC * gB * Cz * Cz + Az * gC2 * Cz + gC * gA * Cr * Cr) * r2
+              + (-0.5 * gC * gA * Cr * Az + 2.0 * gC * gA * Cr * C - 0.5 * gC2 * Br * Az - 0.5 * gC * gB * Cz * Br
+                 + 0.5 * Bz * gB * gC * Cr + gC2 * Br * C + 0.5 * Bz * gC2 * Ar + 0.5 * Ar * gA * gC * Cz) * r
+              + 0.5 * Br * gC * gA * Ar + 0.5 * Br * gB * gA * Cr + 0.5 * gB2 * Bz * Cz + 0.5 * Az * gA * gB * Cz
+              + 0.5 * gA2 * Ar * Cr + 0.5 * Br * Br * gB * gC + gC * gA * C * C - 0.5 * gC * gA * C * Az
+              + 0.5 * Az * gB * gC * Bz + 0.5 * Az * Az * gA * gC + 0.5 * Bz * gB * gC * C
+              + (0.5 * gH2 * C * Hr + 0.5 * gB * gA * Br * C - 0.5 * gH2 * Hz * Hr + 0.5 * gA2 * Ar * C) * ir
+              + 0.5 * Hz * gH2 * lambda;
+        RA += Az * gC2 * Cr * r3
+              + (-0.5 * Az * Az * gC2 + Az * gC2 * C + gB * gA * Cr * Cr + 0.5 * Ar * gC2 * Br + Ar * gC * gA * Cr
+                 + gB * gC * Cr * Br) * r2
+              + (-Az * gB * gA * Cr + 2.0 * gB * gA * Cr * C + gB * gC * C * Br + Ar * gC * gA * C) * r
+              + 0.25 * gH2 * Hr * Hr - gB * gA * C * Az + 0.25 * gB2 * Br * Br + gB * gA * C * C
+              + 0.25 * Ar * Ar * gA2 + 0.5 * gB * gA * Az * Az;
+        RB += Br * gC2 * r3 * Cz
+              + (gA * gB * Cz * Cz + 0.5 * Az * gC2 * Bz - 0.5 * gC2 * Br * Br + Bz * gC * gB * Cz + gA * gC * Cz * Az) * r2
+              - Br * gA * gB * Cz * r + 0.25 * gA2 * Az * Az + 0.5 * gB * gA * Br * Br + 0.25 * gH2 * Hz * Hz
+              + 0.25 * gB2 * Bz * Bz;
+        RH += (-0.5 * C * gC * gH * Hr + gH * gC * Hz * Hr) * r + 0.5 * gH * Hz * Hz * gB + 1.5 * gC * Hz
+              + 0.5 * gH * gA * Hr * Hr - 0.5 * C * gB * gH * Hz - 0.5 * gH * A * gC * Hz - C * gC - 0.5 * gC * Hz
+              + 0.5 * (2.0 * gA - gH - gH * A * gA) * Hr * ir + gH * lambda * (1.0 - gA * H);
+        RC += gC2 * Cr * Cz * r3 + (-0.5 * Az * gC2 * Cz + gC2 * C * Cz - 0.5 * gC2 * Br * Cr) * r2
+              + (-0.5 * gC2 * Br * C + 0.25 * Bz * gC2 * Ar + 0.5 * Ar * gA * gC * Cz + 0.5 * gC * gB * Cz * Br
+                 + 0.75 * gC2 * Br * Az + 0.5 * Bz * gB * gC * Cr + 0.5 * gC * gA * Cr * Az) * r
+              + 0.5 * Az * gA * gB * Cz + 0.5 * Bz * gB * gC * C + 0.5 * gC * gA * C * Az + 0.5 * Br * gB * gA * Cr
+              + (0.25 * gH2 * Hz * Hr + 0.5 * gB * gA * Br * C + 0.25 * gA2 * Ar * Az + 0.25 * Bz * gB2 * Br
+                 - 0.5 * gB * gA * Az * Br) * ir;
+      }
+      RSCAL = (gA * RA + gB * RB + gH * RH + 2.0 * r2 * gC * RC) * ipsi4;
 
-    // ---- divergence of KT (calc_DIVKTA.f90:573-636)
-    {
-      const double Ar = dA.r, Az = dA.z, Br = dB.r, Bz = dB.z, Hr = dH.r, Hz = dH.z, Cr = dC.r, Cz = dC.z;
-      const double KAr = dKA.r, KAz = dKA.z, KBr = dKB.r, KBz = dKB.z, KCr = dKC.r, KCz = dKC.z;
-      const double gA3 = gA2 * gA, gB3 = gB2 * gB, gC3 = gC2 * gC;
-      DIV_r = -gC3 * Cz * KTC * r5 - gA * gC2 * KTC * Cr * r4
-        + (gC2 * KCz - 0.5 * gC3 * Bz * KTA - gA * gC2 * KTC * C - 0.5 * gC3 * Az * KTB - gA * Cz * gC2 * KTA
-           - gA * Cz * gC * gB * KTC - gC2 * Cz * gB * KTB - gC2 * Bz * gB * KTC - gA * Az * gC2 * KTC) * r3
-        + (2.0 * gA * gC * KCr - Delta_r * gC * KTC - gC * Br * gB * gA * KTC - gB * gA2 * KTC * Cr
-           - Ar * gA2 * gC * KTC - gC * gA2 * KTA * Cr + gC2 * KBr - gA * gB * gC * KTB * Cr
-           - 0.5 * gA * Ar * gC2 * KTB - 0.5 * gC2 * Br * gA * KTA) * r2
-        + (2.0 * gA * gC * KTC + gA * gC * KAz - gC * gA2 * KTA * C + gC * gH * KTC - 0.5 * gH2 * KTH * gC * Hz
-           - Delta_z * gC * KTB - gB * gA2 * KTC * C - 0.5 * gA2 * Az * gC * KTA + gC * gB * KBz + gA * gB * KCz
-           - Delta_z * gA * KTC - gA * gB * gC * KTB * C - gC * gH2 * KTH * C - 0.5 * gC * Bz * gB2 * KTB) * r
-        - 0.5 * gA * Br * gB2 * KTB - 0.5 * gH2 * KTH * gA * Hr - Delta_r * gA * KTA + gA2 * KAr
-        - 0.5 * gA3 * Ar * KTA + r * gH * (gA * Alambda + gH * KTH * C * gC);
-      DIV_z = -r5 * gC3 * KTC * Cr + (-gC3 * KTC * C - Cz * gC2 * gB * KTC) * r4
-        + (gC2 * KCr - gB * gC2 * KTB * Cr - gC * gB * gA * KTC * Cr - 0.5 * gC3 * Br * KTA - gC2 * gA * KTA * Cr
-           - 0.5 * Ar * gC3 * KTB - Ar * gA * gC2 * KTC - gB * Br * gC2 * KTC) * r3
-        + (-gB * Cz * gC * gA * KTA + gC2 * KTC - Bz * gC * gB2 * KTC - Delta_z * gC * KTC - gC * Cz * gB2 * KTB
-           + gC2 * KAz + 2.0 * gC * gB * KCz - gC2 * gA * KTA * C - Cz * gB2 * gA * KTC - 0.5 * Az * gB * gC2 * KTB
-           - gC * Az * gB * gA * KTC - gB * gC2 * KTB * C - 0.5 * gB * Bz * gC2 * KTA - gC * gB * gA * KTC * C) * r2
-        + (gB * gC * KBr + gB * gA * KCr + gC * gA * KAr - 0.5 * gH2 * KTH * gC * Hr - Delta_r * gB * KTC
-           - Delta_r * gC * KTA - 0.5 * gB2 * Br * gC * KTB - 0.5 * gC * Ar * gA2 * KTA) * r
-        + gC * gH * KTA + gB * gA * KTC - 0.5 * Bz * gB3 * KTB + gB2 * KBz - Delta_z * gB * KTB
-        - gC * gH2 * KTH * A - 0.5 * gB * Az * gA2 * KTA - 0.5 * gH2 * KTH * gB * Hz - gC * gH * KTH
-        + gB * gH * KTC - gB * gH2 * KTH * C;
-      DIVA_r = -r5 * gC3 * Cz * KTC + (-gC3 * Cr * KTB - gC3 * Br * KTC - 3.0 * gC2 * Cr * gA * KTC) * r4
-        + (-gC3 * C * KTB - 2.0 * gA * Cz * gC2 * KTA - 3.0 * gC2 * C * gA * KTC + gC2 * KCz - gC2 * Az * gA * KTC
-           - 3.0 * gA * Cz * gC * gB * KTC - 2.0 * gC2 * Bz * gB * KTC - gC3 * Bz * KTA - gC2 * Cz * gB * KTB) * r3
-        + (-gA * Br * gC2 * KTA - gA * gB * gC * KTB * Cr + 2.0 * gA * gC * KCr - 2.0 * gC * Cr * gA2 * KTA
-           - Delta_r * gC * KTC + gC2 * KBr - gC2 * Br * gB * KTB - 2.0 * gA2 * Ar * gC * KTC - gA * Ar * gC2 * KTB
-           - gB * gA2 * KTC * Cr - gA * Br * gC * gB * KTC) * r2
-        + (-gA * Cz * gB2 * KTB - gB * gA2 * KTC * C - gC * Bz * gB2 * KTB + 2.0 * gA * gC * KTC + gC * gB * KBz
-           + gC * gH * KTC + gA * gC * KAz + gA * gB * KCz - gA * Az * gB * gC * KTB - gC * gH2 * KTH * C
-           - Delta_z * gC * KTB - gA2 * Az * gB * KTC - 2.0 * gC * C * gA2 * KTA - Delta_z * gA * KTC
-           - gA2 * Az * gC * KTA - gA * gB * gC * KTB * C) * r
-        - Ar * gA3 * KTA - Delta_r * gA * KTA + gA2 * KAr + r * gA * gH * (Alambda - lambda * gH * KTH);
-      DIVA_z = -r5 * gC3 * KTC * Cr + (-3.0 * Cz * gC2 * gB * KTC - Az * gC3 * KTC - Cz * gC3 * KTA - gC3 * KTC * C) * r4
-        + (-3.0 * gC * gB * gA * KTC * Cr - 2.0 * Ar * gA * gC2 * KTC + gC2 * KCr - gB * Br * gC2 * KTC
-           - Ar * gC3 * KTB - 2.0 * gB * gC2 * KTB * Cr - gC2 * gA * KTA * Cr) * r3
-        + (-gC * Az * gB * gA * KTC - gC2 * gA * KTA * C - 2.0 * Bz * gC * gB2 * KTC - Az * gC2 * gA * KTA
-           - Az * gB * gC2 * KTB - 3.0 * gC * gB * gA * KTC * C - 2.0 * gC * Cz * gB2 * KTB - Delta_z * gC * KTC
-           - Cz * gB2 * gA * KTC + gC2 * KTC + gC2 * KAz + 2.0 * gC * gB * KCz - gB * Cz * gC * gA * KTA
-           - gB * Bz * gC2 * KTA - 2.0 * gB * gC2 * KTB * C) * r2
-        + (-Delta_r * gB * KTC + gB * gA * KCr + gC * gA * KAr + gB * gC * KBr - gB2 * Br * gC * KTB
-           - Delta_r * gC * KTA - gB2 * Br * gA * KTC - gC * Ar * gA2 * KTA - gB * Br * gC * gA * KTA
-           - gB * gA2 * KTA * Cr) * r
-        - Delta_z * gB * KTB - gB * gH2 * KTH * C + gB2 * KBz - gC * gH2 * KTH * A - gB * gA2 * KTA * C
-        + gB * gH * KTC + gC * gH * KTA + gB * gA * KTC - Bz * gB3 * KTB;
+    }
+    if constexpr (STAGE != 11) {
+      // ---- divergence of KT (calc_DIVKTA.f90:573-636)
+      {
+        const double Ar = dA.r, Az = dA.z, Br = dB.r, Bz = dB.z, Hr = dH.r, Hz = dH.z, Cr = dC.r, Cz = dC.z;
+        const double KAr = dKA.r, KAz = dKA.z, KBr = dKB.r, KBz = dKB.z, KCr = dKC.r, KCz = dKC.z;
+        const double gA3 = gA2 * gA, gB3 = gB2 * gB, gC3 = gC2 * gC;
+        DIV_r = -gC3 * Cz * KTC * r5 - gA * gC2 * KTC * Cr * r4
+          + (gC2 * KCz - 0.5 * gC3 * Bz * KTA - gA * gC2 * KTC * C - 0.5 * gC3 * Az * KTB - gA * Cz * gC2 * KTA
+             - gA * Cz * gC * gB * KTC - gC2 * Cz * gB * KTB - gC2 * Bz * gB * KTC - gA * Az * gC2 * KTC) * r3
+          + (2.0 * gA * gC * KCr - Delta_r * gC * KTC - gC * Br * gB * gA * KTC - gB * gA2 * KTC * Cr
+             - Ar * gA2 * gC * KTC - gC * gA2 * KTA * Cr + gC2 * KBr - gA * gB * gC * KTB * Cr
+             - 0.5 * gA * Ar * gC2 * KTB - 0.5 * gC2 * Br * gA * KTA) * r2
+          + (2.0 * gA * gC * KTC + gA * gC * KAz - gC * gA2 * KTA * C + gC * gH * KTC - 0.5 * gH2 * KTH * gC * Hz
+             - Delta_z * gC * KTB - gB * gA2 * KTC * C - 0.5 * gA2 * Az * gC * KTA + gC * gB * KBz + gA * gB * KCz
+             - Delta_z * gA * KTC - gA * gB * gC * KTB * C - gC * gH2 * KTH * C - 0.5 * gC * Bz * gB2 * KTB) * r
+          - 0.5 * gA * Br * gB2 * KTB - 0.5 * gH2 * KTH * gA * Hr - Delta_r * gA * KTA + gA2 * KAr
+          - 0.5 * gA3 * Ar * KTA + r * gH * (gA * Alambda + gH * KTH * C * gC);
+        DIV_z = -r5 * gC3 * KTC * Cr + (-gC3 * KTC * C - Cz * gC2 * gB * KTC) * r4
+          + (gC2 * KCr - gB * gC2 * KTB * Cr - gC * gB * gA * KTC * Cr - 0.5 * gC3 * Br * KTA - gC2 * gA * KTA * Cr
+             - 0.5 * Ar * gC3 * KTB - Ar * gA * gC2 * KTC - gB * Br * gC2 * KTC) * r3
+          + (-gB * Cz * gC * gA * KTA + gC2 * KTC - Bz * gC * gB2 * KTC - Delta_z * gC * KTC - gC * Cz * gB2 * KTB
+             + gC2 * KAz + 2.0 * gC * gB * KCz - gC2 * gA * KTA * C - Cz * gB2 * gA * KTC - 0.5 * Az * gB * gC2 * KTB
+             - gC * Az * gB * gA * KTC - gB * gC2 * KTB * C - 0.5 * gB * Bz * gC2 * KTA - gC * gB * gA * KTC * C) * r2
+          + (gB * gC * KBr + gB * gA * KCr + gC * gA * KAr - 0.5 * gH2 * KTH * gC * Hr - Delta_r * gB * KTC
+             - Delta_r * gC * KTA - 0.5 * gB2 * Br * gC * KTB - 0.5 * gC * Ar * gA2 * KTA) * r
+          + gC * gH * KTA + gB * gA * KTC - 0.5 * Bz * gB3 * KTB + gB2 * KBz - Delta_z * gB * KTB
+          - gC * gH2 * KTH * A - 0.5 * gB * Az * gA2 * KTA - 0.5 * gH2 * KTH * gB * Hz - gC * gH * KTH
+          + gB * gH * KTC - gB * gH2 * KTH * C;
+        DIVA_r = -r5 * gC3 * Cz * KTC + (-gC3 * Cr * KTB - gC3 * Br * KTC - 3.0 * gC2 * Cr * gA * KTC) * r4
+          + (-gC3 * C * KTB - 2.0 * gA * Cz * gC2 * KTA - 3.0 * gC2 * C * gA * KTC + gC2 * KCz - gC2 * Az * gA * KTC
+             - 3.0 * gA * Cz * gC * gB * KTC - 2.0 * gC2 * Bz * gB * KTC - gC3 * Bz * KTA - gC2 * Cz * gB * KTB) * r3
+          + (-gA * Br * gC2 * KTA - gA * gB * gC * KTB * Cr + 2.0 * gA * gC * KCr - 2.0 * gC * Cr * gA2 * KTA
+             - Delta_r * gC * KTC + gC2 * KBr - gC2 * Br * gB * KTB - 2.0 * gA2 * Ar * gC * KTC - gA * Ar * gC2 * KTB
+             - gB * gA2 * KTC * Cr - gA * Br * gC * gB * KTC) * r2
+          + (-gA * Cz * gB2 * KTB - gB * gA2 * KTC * C - gC * Bz * gB2 * KTB + 2.0 * gA * gC * KTC + gC * gB * KBz
+             + gC * gH * KTC + gA * gC * KAz + gA * gB * KCz - gA * Az * gB * gC * KTB - gC * gH2 * KTH * C
+             - Delta_z * gC * KTB - gA2 * Az * gB * KTC - 2.0 * gC * C * gA2 * KTA - Delta_z * gA * KTC
+             - gA2 * Az * gC * KTA - gA * gB * gC * KTB * C) * r
+          - Ar * gA3 * KTA - Delta_r * gA * KTA + gA2 * KAr + r * gA * gH * (Alambda - lambda * gH * KTH);
+        DIVA_z = -r5 * gC3 * KTC * Cr + (-3.0 * Cz * gC2 * gB * KTC - Az * gC3 * KTC - Cz * gC3 * KTA - gC3 * KTC * C) * r4
+          + (-3.0 * gC * gB * gA * KTC * Cr - 2.0 * Ar * gA * gC2 * KTC + gC2 * KCr - gB * Br * gC2 * KTC
+             - Ar * gC3 * KTB - 2.0 * gB * gC2 * KTB * Cr - gC2 * gA * KTA * Cr) * r3
+          + (-gC * Az * gB * gA * KTC - gC2 * gA * KTA * C - 2.0 * Bz * gC * gB2 * KTC - Az * gC2 * gA * KTA
+             - Az * gB * gC2 * KTB - 3.0 * gC * gB * gA * KTC * C - 2.0 * gC * Cz * gB2 * KTB - Delta_z * gC * KTC
+             - Cz * gB2 * gA * KTC + gC2 * KTC + gC2 * KAz + 2.0 * gC * gB * KCz - gB * Cz * gC * gA * KTA
+             - gB * Bz * gC2 * KTA - 2.0 * gB * gC2 * KTB * C) * r2
+          + (-Delta_r * gB * KTC + gB * gA * KCr + gC * gA * KAr + gB * gC * KBr - gB2 * Br * gC * KTB
+             - Delta_r * gC * KTA - gB2 * Br * gA * KTC - gC * Ar * gA2 * KTA - gB * Br * gC * gA * KTA
+             - gB * gA2 * KTA * Cr) * r
+          - Delta_z * gB * KTB - gB * gH2 * KTH * C + gB2 * KBz - gC * gH2 * KTH * A - gB * gA2 * KTA * C
+          + gB * gH * KTC + gC * gH * KTA + gB * gA * KTC - Bz * gB3 * KTB;
+      }
     }
   }
-  if constexpr (STAGE == 1) {
+  // STAGE 11 / 12: the two halves of STAGE 1 (D_i D_j alpha and the Ricci
+  // tensor; div KT), for kernels that run them one after the other
+  if constexpr (STAGE == 1 || STAGE == 11 || STAGE == 12) {
     double* const ax = gaux + k;
-    ax[AUX_d2A * pl] = d2A; ax[AUX_d2B * pl] = d2B; ax[AUX_d2C * pl] = d2C; ax[AUX_Lapla * pl] = Lapla;
-    ax[AUX_RA * pl] = RA; ax[AUX_RB * pl] = RB; ax[AUX_RC * pl] = RC; ax[AUX_RSCAL * pl] = RSCAL;
-    ax[AUX_DIV_r * pl] = DIV_r; ax[AUX_DIV_z * pl] = DIV_z; ax[AUX_DIVA_r * pl] = DIVA_r; ax[AUX_DIVA_z * pl] = DIVA_z;
+    if constexpr (STAGE != 12) {
+      ax[AUX_d2A * pl] = d2A; ax[AUX_d2B * pl] = d2B; ax[AUX_d2C * pl] = d2C; ax[AUX_Lapla * pl] = Lapla;
+      ax[AUX_RA * pl] = RA; ax[AUX_RB * pl] = RB; ax[AUX_RC * pl] = RC; ax[AUX_RSCAL * pl] = RSCAL;
+    }
+    if constexpr (STAGE != 11) {
+      ax[AUX_DIV_r * pl] = DIV_r; ax[AUX_DIV_z * pl] = DIV_z; ax[AUX_DIVA_r * pl] = DIVA_r; ax[AUX_DIVA_z * pl] = DIVA_z;
+    }
     return;
   }
 

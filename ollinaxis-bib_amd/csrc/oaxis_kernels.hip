@@ -104,7 +104,17 @@ __device__ __forceinline__ const Item& find_item(const Item* __restrict__ items,
 // vacuum fields) and passes them through the per-level buffer gaux; phase 2
 // forms every source, the dissipation, the radiative boundaries and the RK
 // update.  Kept apart, the phases need at most the registers of the larger
-// one, so the kernel runs at two waves per SIMD without spilling.
+// one.
+// Occupancy of the RHS kernel.  With the Gamma-driver or the scalar field the
+// staged tile (16 or 20 fields, 62-77 KB) allows two blocks per CU, and the
+// two phases fit 256 VGPRs: two waves per SIMD.  In vacuum the tile (12
+// fields, 46.5 KB) allows three blocks, and phase 1 split in two halves fits
+// 168 VGPRs: three waves per SIMD.
+template <bool SHIFT, bool SCALAR>
+constexpr bool kSplitPhase1 = !SHIFT && !SCALAR;
+template <bool SHIFT, bool SCALAR>
+constexpr int kRhsBlocksPerCu = kSplitPhase1<SHIFT, SCALAR> ? 3 : 2;
+
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
 __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, double* tile) {
   constexpr int nf = nslots(SHIFT, SCALAR);
@@ -129,10 +139,24 @@ __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, doub
   // only, so no synchronisation); the memory barrier keeps the compiler from
   // carrying phase-1 values or loads into phase 2, so the two phases do not
   // add up in registers
-  if (interior_tile(G, i0, j0))
-    rhs_point<false, false, PHIMODE, true, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
-  else
-    rhs_point<false, false, PHIMODE, false, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+  if constexpr (kSplitPhase1<SHIFT, SCALAR>) {
+    // vacuum: phase 1 in its two halves (D_i D_j alpha + Ricci, then div KT),
+    // so that no part needs more than 168 VGPRs: three waves per SIMD
+    if (interior_tile(G, i0, j0)) {
+      rhs_point<false, false, PHIMODE, true, 11>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+      asm volatile("" ::: "memory");
+      rhs_point<false, false, PHIMODE, true, 12>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+    } else {
+      rhs_point<false, false, PHIMODE, false, 11>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+      asm volatile("" ::: "memory");
+      rhs_point<false, false, PHIMODE, false, 12>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+    }
+  } else {
+    if (interior_tile(G, i0, j0))
+      rhs_point<false, false, PHIMODE, true, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+    else
+      rhs_point<false, false, PHIMODE, false, 1>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
+  }
   asm volatile("" ::: "memory");
   rhs_point<SHIFT, SCALAR, PHIMODE, false, 2>(G, it.P, S, V, gmem(it.yp), gmem(it.acc), gmem(it.yout), gaux, i, j);
 }
@@ -149,7 +173,7 @@ __device__ __forceinline__ int xcd_block(int b, int total) {
 }
 
 template <bool SHIFT, bool SCALAR, bool PHIMODE>
-__global__ void __launch_bounds__(kTile * kTile, 2) k_rhs_table(const RhsItem* __restrict__ items, int n,
+__global__ void __launch_bounds__(kTile * kTile, (kRhsBlocksPerCu<SHIFT, SCALAR>)) k_rhs_table(const RhsItem* __restrict__ items, int n,
                                                                 int xcd_remap) {
   extern __shared__ double tile[];
   const int b = xcd_remap ? xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))
