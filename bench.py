@@ -52,16 +52,18 @@ def bytes_per_update(params, nfields):
     return 8.0 * nfields * (2 + 3 * (it - 1)), it
 
 
-def measured_traffic(workload):
-    """HBM bytes per k_rhs launch for this workload from the committed
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+def measured_traffic(workload, algorithmic_bytes):
+    """HBM bytes per RHS launch for this workload: the copy-calibrated
+    FETCH_SIZE + WRITE_SIZE per point measured for it (committed summary),
+    relative to its algorithmic bytes, times the launch's algorithmic bytes;
+    None when no measurement is committed."""
     path = os.path.join(ROOT, "profiles", "k_rhs_hbm_traffic.md")
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload:
-                    return float(parts[2])
+                if len(parts) == 3 and parts[0] == "traffic_ratio" and parts[1] == workload:
+                    return float(parts[2]) * algorithmic_bytes
     except OSError:
         pass
     return None
@@ -208,7 +210,8 @@ def run_batched(args):
                    "plain_value": plain_value, "plain_ms_per_step": plain_wall / args.steps * 1e3,
                    "batched_vs_plain": value / plain_value, "schedule": schedule},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, rhs_bytes / rhs_n if rhs_n else 0.0),
                      "kernel": "k_rhs_table", "avg_launch_ms": rhs_ms / rhs_n if rhs_n else 0.0,
                      "launches": rhs_n,
                      "algorithmic_bytes_per_launch": rhs_bytes / rhs_n if rhs_n else 0.0},
@@ -329,7 +332,7 @@ def main():
                    "integrator": "rk4" if params.integrator == 1 else "icn",
                    "point_updates": updates, "device_ms": device_ms, "schedule": schedule},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, bytes_per_launch),
                      "kernel": "k_rhs_table", "avg_launch_ms": rhs_ms, "launches": rhs_n,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
     }
