@@ -1144,6 +1144,37 @@ OA_HD double apply(const Cell& c, const double* __restrict__ f, const KGrid& G) 
   return out;
 }
 
+// apply() for N fields of one level at the same cell: one branch on the
+// cell kind, then each field with apply()'s own order of operations.
+template <int N>
+OA_HD void apply_many(const Cell& c, const double* __restrict__ y, long plane, const int (&slot)[N], double (&out)[N],
+                      const KGrid& G) {
+  if (c.copy) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) out[q] = y[slot[q] * plane + IDX(G, c.i0, c.j0)];
+    return;
+  }
+  if (c.lin) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const double* f = y + slot[q] * plane;
+      double o = 0.0;
+      for (int a = 0; a <= 1; ++a)
+        for (int b = 0; b <= 1; ++b) o += c.wr[a + 1] * c.wz[b + 1] * f[IDX(G, c.i0 + a, c.j0 + b)];
+      out[q] = o;
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    const double* f = y + slot[q] * plane;
+    double o = 0.0;
+    for (int a = -1; a <= 2; ++a)
+      for (int b = -1; b <= 2; ++b) o += c.wr[a + 1] * c.wz[b + 1] * f[IDX(G, c.i0 + a, c.j0 + b)];
+    out[q] = o;
+  }
+}
+
 // finebound(box 0, level, dtw, .true.).  The fine boundary points and the
 // coarse cells they interpolate from are fixed per level and zone depth, so
 // fb_geometry resolves them once into FbPoint records; finebound_point then
@@ -1207,8 +1238,9 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
                            const double* __restrict__ yc, double* __restrict__ yf,
                            const double* __restrict__ rR0, const double* __restrict__ rR1,
                            const double* __restrict__ zL0, const double* __restrict__ zL1,
-                           const double* __restrict__ zR0, const double* __restrict__ zR1, const FbPoint& fp,
+                           const double* __restrict__ zR0, const double* __restrict__ zR1, const FbPoint& fp_in,
                            int grp) {
+  const FbPoint fp = fp_in;   // the whole record in one go
   if (fp.which < 0) return;
   const int g = Gf.g, i = fp.i, j = fp.j;
   const Cell& c = fp.c;
@@ -1229,9 +1261,27 @@ OA_HD void finebound_point(const KGrid& Gf, const KGrid& Gc, const KPhys& P, con
     return T.c0 * aux2 + (T.same ? 0.0 : T.c1 * b0 + T.c2 * b1);
   };
   if (grp == 0) {
-    // the fields coupled by the projection (chi/phi, factordet) go together
+    // the fields coupled by the projection (chi/phi, factordet) go together;
+    // the six interpolations share one branch on the cell kind and issue
+    // their loads together (value() for each field in turn would wait for
+    // each field's loads before starting the next)
+    constexpr int kN = 6;
+    const int slot[kN] = {S_chi, S_phi, S_A, S_B, S_H, S_C};
+    double v[kN];
+    apply_many<kN>(c, yc, pc, slot, v, Gc);
+    double b0[kN], b1[kN];
+#pragma unroll
+    for (int q = 0; q < kN; ++q) {
+      b0[q] = strip_ok ? s0[slot[q] * sper + sidx] : 0.0;
+      b1[q] = (strip_ok && T.border != 1) ? s1[slot[q] * sper + sidx] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kN; ++q) {
+      if (T.border == 1) v[q] = T.c_aux * v[q] + (T.same ? 0.0 : T.c_p * b0[q]);
+      else v[q] = T.c0 * v[q] + (T.same ? 0.0 : T.c1 * b0[q] + T.c2 * b1[q]);
+    }
     const double r = coord_r(Gf, i);
-    Proj qp{value(S_chi), value(S_phi), value(S_A), value(S_B), value(S_H), value(S_C)};
+    Proj qp{v[0], v[1], v[2], v[3], v[4], v[5]};
     project(qp, r, time0 != 0, P.chimethod, SCALAR);
     put(yf + S_chi * pf, Gf, i, j, qp.chi, 1.0, 1.0);
     if (time0 || !P.chimethod) put(yf + S_phi * pf, Gf, i, j, qp.phi, 1.0, 1.0);   // phi is not kept after the first step
