@@ -60,6 +60,7 @@ enum { OA_IDATA_CHECKPOINT = 0, OA_IDATA_MINKOWSKI, OA_IDATA_SCHWARZSCHILD,
        OA_IDATA_BOSONSTAR, OA_IDATA_ROTBOSON };
 enum { OA_BOUND_NONE = 0, OA_BOUND_STATIC, OA_BOUND_FLAT, OA_BOUND_RADIATIVE };
 enum { OA_INTERP_BILINEAR = 1, OA_INTERP_BICUBIC = 3 };
+enum { OA_AH_SHOOT = 0, OA_AH_SPECTRAL = 1, OA_AH_BOTH = 2 };
 enum { OA_EULERIAN = 0, OA_LAGRANGIAN = 1 };
 enum { OA_BSSN = 0, OA_Z4C = 1 };
 enum { OA_DYNAMIC = 0, OA_BACKGROUND = 1 };
@@ -117,9 +118,11 @@ typedef struct oa_params {
   int    N_BH;
   double BH1mass, BH1spin, BH1Z, BH2mass, BH2spin, BH2Z;
   int    duplicate_bhs;
-  /* horizons: only what the serving loop uses (param.f90:540-562) */
+  /* horizons (param.f90:540-562) */
   int    ahfind, ahfind_every;
   double ahafter;
+  int    ahmove, N_ah, ahmethod;  /* ahmethod: OA_AH_*                     */
+  double z_ah1, z_ah2, ahzinit, ahrmax, ahepsilon_shoot;
   /* Brill waves (param.f90:572-576) */
   double brill_a0, brill_sr0, brill_sz0;
   /* matter (param.f90:615-648) */
@@ -256,6 +259,38 @@ int oa_constraints(oa_handle* h, int box, int level);
  * reference's serial loop, so nm2 agrees to rounding).  A NaN in the range
  * returns OA_ERR_STATE (the reference stops the run there). */
 int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out);
+/* One apparent-horizon search: what ahshoot (src/geometry/horizon_finder.f90:
+ * 246-1103) finds for horizon k and writes to its ah_area / ah_mass files. */
+typedef struct oa_horizon {
+  int    found;      /* ahfound(k) after the search.  As in the reference it keeps
+                        its previous value when the bisection converges on a
+                        non-horizon within dzlocal of the centre (:797-845).     */
+  int    ntheta;     /* Ntheta: accepted steps of the last integration          */
+  int    shots;      /* integrations of the horizon ODE this search ran         */
+  int    errorflag;  /* errorflag of the last integration                       */
+  double zcenter;    /* centre on the axis, zah(k) (:100-110)                   */
+  double zinit;      /* axis position where the shooting stopped (:797-815)     */
+  double dradlast;   /* dh/dtheta at the end of the last integration            */
+  double area;       /* ah_area: 0 unless found; doubled with eqsym (:989-995)  */
+  double radius;     /* RAH = sqrt(area/pi)/2 (:997)                            */
+  double mass;       /* MAH = RAH/2: no angular momentum in scope (:1065-1076)  */
+} oa_horizon;
+
+/* horizon_finder (horizon_finder.f90:2-237) for horizon k (0 <= k < N_ah,
+ * centre 0, z_ah1, z_ah2) with ahmethod = shoot, on the current state of
+ * box 0 (every level projected): replaces the reference's call at
+ * evolve.f90:251 and :429-432.  The metric quantities the horizon ODE
+ * interpolates are formed on the device for every level, and the shots of
+ * the search (the march along the axis and the bisection) are integrated on
+ * the device, many per launch; the sequence of shots and every decision
+ * between them are the reference's.  ahmove = .true. and the spectral
+ * method return OA_ERR_UNSUPPORTED. */
+int oa_ahfind(oa_handle* h, int k, oa_horizon* out);
+/* (theta_i, h(theta_i)), i = 0..ntheta, of the last integration of horizon k
+ * (what ahshoot writes to ah_radius_SH, :956-966; h = 0 when not found).
+ * cap is the room in theta and rad; *n receives ntheta + 1. */
+int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n);
+
 /* Device memory (bytes) oa_create allocates for these parameters: what a
  * scheduler admitting spacetimes by HBM footprint needs. */
 int oa_params_footprint(const oa_params* p, double* bytes);

@@ -35,6 +35,8 @@ struct Level {
   long fbn[2] = {0, 0};
 };
 
+struct AhState;                                     // apparent-horizon finder (oaxis_ah.hip)
+
 }  // namespace oax
 
 struct oa_batch;
@@ -52,6 +54,7 @@ struct oa_handle {
   oax::KPhys phys{};
   oax::Plan plan;                    // launches of this handle's own calls
   double* norms_part = nullptr;      // partial records of the 0D norms
+  oax::AhState* ah = nullptr;        // horizon finder state (on demand)
   // timing
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   double last_ms = 0.0;
@@ -80,5 +83,6 @@ void finish_level_step(oa_handle* h, int level);   // advance the level clock, c
 // onestep(level) of every spacetime of `all` that has the level, in lockstep,
 // appended to `plan` (oa_plan.cpp)
 int plan_level(Plan& plan, const std::vector<oa_handle*>& all, int level);
+void ah_release(oa_handle* h);                       // free the horizon finder's device memory
 
 }  // namespace oax

@@ -1,0 +1,946 @@
+// Apparent-horizon finder with the shooting method on the device.
+//
+// Reference: src/geometry/horizon_finder.f90 -- horizon_finder (:2-237),
+// ahshoot (:246-1103), ah_source (:1983-2752) -- with ahmethod = shoot,
+// ahmove = .false. and no angular momentum (no C1/C2 arrays in this hot
+// path), for box 0.
+//
+// The reference integrates the horizon ODE h'' = sdh(h, h', theta) once per
+// "shot" from an axis point zinit, marching zinit down the axis by half a
+// local cell until the end slope dh/dtheta changes sign, then bisecting.
+// Every shot is a serial adaptive RK4 over ~10^2-10^3 steps whose sources
+// interpolate 20 metric quantities, so one shot is a long dependent chain:
+// this file runs one shot per lane and many shots per launch.
+//
+//  1. k_ah_fields: one streaming pass per level forms the derived planes the
+//     ODE interpolates (psi, the conformal-factor and metric first
+//     derivatives, the physical extrinsic curvature), with their parity
+//     images in the ghost zones, as the auxiliary arrays hold them.
+//  2. k_ah_shoot: one lane integrates one shot (zinit) to the end.
+//  3. The host walks the reference's control flow (march, bracket, bisection,
+//     restart; ahshoot :434-855) over a table of shot results.  When the walk
+//     needs a shot it has not got, it launches the whole speculative
+//     continuation at once: during the march, the next positions down the
+//     axis; during the bisection, the full tree of the next kAhDepth
+//     bisection steps.  The walk then takes exactly the reference's path
+//     through those results, so the sequence of shots and every decision
+//     are the reference's; unused shots are discarded.
+//
+// Floating-point contraction is off in this file, so the sources and the
+// interpolation round like the oracle's IEEE build of the same expressions.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "oa_internal.h"
+#include "oa_state.h"
+#define OA_HD __device__ __forceinline__
+#define OA_HHD __host__ __device__ inline
+#include "oaxis_device.h"
+
+namespace oax {
+namespace {
+
+constexpr int kAhMaxLevels = 16;
+constexpr int kAhNthetaMax = 50000;                 // Nthetamax (:350)
+// Accepted plus rejected steps of one shot.  The reference bounds only the
+// accepted ones; rejections halve the step until it is accepted, so this
+// bound is reached only by a shot the reference would integrate for hours.
+constexpr int kAhMaxSteps = 4 * kAhNthetaMax;
+constexpr int kAhScanChunk = 1024;                  // march positions per launch
+constexpr int kAhDepth = 8;                         // bisection steps per launch
+constexpr int kShotBlock = 64;
+
+// interpolated quantities, in ah_source's order
+enum AhField {
+  AF_psi = 0, AF_Drphi, AF_Dzphi,
+  AF_A, AF_DrA, AF_DzA, AF_B, AF_DrB, AF_DzB, AF_C, AF_DrC, AF_DzC, AF_H, AF_DrH, AF_DzH,
+  AF_trK, AF_KA, AF_KB, AF_KC, AF_KH, AF_N
+};
+// derived planes k_ah_fields writes (the other AF_* are state planes)
+enum AhPlane {
+  AP_psi = 0, AP_Drphi, AP_Dzphi, AP_DrA, AP_DzA, AP_DrB, AP_DzB, AP_DrC, AP_DzC, AP_DrH, AP_DzH,
+  AP_KA, AP_KB, AP_KC, AP_KH, AP_N
+};
+
+struct AhLevel {
+  KGrid G;
+  double rmin, rmax, zmin, zmax;      // rminl .. zmaxl (evolve.f90:80-95)
+  const double* p[AF_N];              // plane of every interpolated quantity
+};
+
+struct AhCtx {
+  int nlev, eqsym, interporder, method;
+  double ahrmax;                      // clamped to 0.9 as ahshoot does (:367-373)
+  AhLevel L[kAhMaxLevels];
+};
+
+struct ShotIn {
+  const AhCtx* ctx;
+  double zinit, zcenter;
+  double* theta;                      // when not null: accepted points (kAhNthetaMax + 1)
+  double* rad;
+};
+
+struct ShotOut {
+  double dradlast, area;
+  int err, ntheta;
+};
+
+// ------------------------------------------------------------ derived planes
+// psi = exp(phi) (auxiliary_geometry.f90:236-242), Dr/Dz of phi (:176-231),
+// of A, B, C, H (diff1r/diff1z with the metric's parities), and
+// KA..KH = psi4 (KT* + X trK / 3) (:1038-1041) with KTH from tracelessness
+// (:747-756).  PHIMODE: the last auxiliary ran at level clock 0, so the
+// conformal-factor derivatives are those of phi; otherwise of chi.
+template <bool PHIMODE>
+__global__ void __launch_bounds__(256) k_ah_fields(KGrid G, const double* __restrict__ y, int phi_from_chi,
+                                                   double* __restrict__ out) {
+  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > G.Nz || i > G.Nr) return;
+  const long pl = G.plane;
+  const GlobalView V{y, G, nullptr};
+  const double r = (static_cast<double>(i) - 0.5) * G.dr;
+  const double chi = V.fld(S_chi)(i, j);
+  const double phi = phi_from_chi ? -std::log(chi) / 2.0 : V.fld(S_phi)(i, j);
+  const double psi = std::exp(phi);
+  double Drp, Dzp;
+  if constexpr (PHIMODE) {
+    const D1 dp = grad(V.fld(S_phi), G, i, j);
+    Drp = dp.r;
+    Dzp = dp.z;
+  } else {
+    const D1 dc = grad(V.fld(S_chi), G, i, j);
+    Drp = -(dc.r / chi) / 2.0;
+    Dzp = -(dc.z / chi) / 2.0;
+  }
+  const D1 dA = grad(V.fld(S_A), G, i, j), dB = grad(V.fld(S_B), G, i, j);
+  const D1 dC = grad(V.fld(S_C), G, i, j), dH = grad(V.fld(S_H), G, i, j);
+  const double A = V.fld(S_A)(i, j), B = V.fld(S_B)(i, j), C = V.fld(S_C)(i, j), H = V.fld(S_H)(i, j);
+  const double trK = V.fld(S_trK)(i, j), KTA = V.fld(S_KTA)(i, j), KTB = V.fld(S_KTB)(i, j);
+  const double KTC = V.fld(S_KTC)(i, j);
+  const double gA = B * H, gB = A * H, gH = 1.0 / H, gC = -C * H;
+  const double KTH = -(gA * KTA + gB * KTB + 2.0 * r * r * gC * KTC) / gH;
+  const double psi2 = psi * psi, psi4 = psi2 * psi2;
+  const double third = 1.0 / 3.0;
+  put(out + AP_psi * pl, G, i, j, psi, 1.0, 1.0);
+  put(out + AP_Drphi * pl, G, i, j, Drp, -1.0, 1.0);
+  put(out + AP_Dzphi * pl, G, i, j, Dzp, 1.0, -1.0);
+  put(out + AP_DrA * pl, G, i, j, dA.r, -1.0, 1.0);
+  put(out + AP_DzA * pl, G, i, j, dA.z, 1.0, -1.0);
+  put(out + AP_DrB * pl, G, i, j, dB.r, -1.0, 1.0);
+  put(out + AP_DzB * pl, G, i, j, dB.z, 1.0, -1.0);
+  put(out + AP_DrC * pl, G, i, j, dC.r, -1.0, -1.0);
+  put(out + AP_DzC * pl, G, i, j, dC.z, 1.0, 1.0);
+  put(out + AP_DrH * pl, G, i, j, dH.r, -1.0, 1.0);
+  put(out + AP_DzH * pl, G, i, j, dH.z, 1.0, -1.0);
+  put(out + AP_KA * pl, G, i, j, psi4 * (KTA + third * A * trK), 1.0, 1.0);
+  put(out + AP_KB * pl, G, i, j, psi4 * (KTB + third * B * trK), 1.0, 1.0);
+  put(out + AP_KC * pl, G, i, j, psi4 * (KTC + third * C * trK), 1.0, -1.0);
+  put(out + AP_KH * pl, G, i, j, psi4 * (KTH + third * H * trK), 1.0, 1.0);
+}
+
+// ------------------------------------------------------------ interpolation
+// interp (interp.f90:2-270) of every AF_* quantity at one point: one set of
+// Lagrange weights, formed in the reference's order, for all of them.  The
+// caller (ah_source) has already placed (r0, z0) at least one cell inside the
+// level (one cell inside level 0's edges); the stencil rows and columns are
+// clamped to the array, which matters only where a weight is exactly zero.
+__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N]) {
+  const KGrid& G = L.G;
+  const int g = G.g;
+  const double r1g = (static_cast<double>(1 - g) - 0.5) * G.dr;
+  const double z1g = (static_cast<double>(1 - g) - G.zoff) * G.dz;
+  const int i0 = static_cast<int>((r0 - r1g) / G.dr) + 1 - g;
+  const double deltar = r0 - (static_cast<double>(i0) - 0.5) * G.dr;
+  const int j0 = static_cast<int>((z0 - z1g) / G.dz) + 1 - g;
+  const double deltaz = z0 - (static_cast<double>(j0) - G.zoff) * G.dz;
+  auto ci = [&](int i) { return i < 1 - g ? 1 - g : (i > G.Nr ? G.Nr : i); };
+  auto cj = [&](int j) { return j < 1 - g ? 1 - g : (j > G.Nz ? G.Nz : j); };
+  if (deltar == 0.0 && deltaz == 0.0) {
+    const long k = IDX(G, ci(i0), cj(j0));
+#pragma unroll
+    for (int f = 0; f < AF_N; ++f) v[f] = L.p[f][k];
+    return;
+  }
+  const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
+                   ((z0 - L.zmin) < G.dz);
+  const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
+  if (order == OA_INTERP_BILINEAR || lin) {
+    const long k00 = IDX(G, ci(i0), cj(j0)), k10 = IDX(G, ci(i0 + 1), cj(j0));
+    const long k01 = IDX(G, ci(i0), cj(j0 + 1)), k11 = IDX(G, ci(i0 + 1), cj(j0 + 1));
+#pragma unroll
+    for (int f = 0; f < AF_N; ++f) {
+      const double* p = L.p[f];
+      v[f] = p[k00] * (1.0 - auxr) * (1.0 - auxz) + p[k10] * auxr * (1.0 - auxz) + p[k01] * (1.0 - auxr) * auxz +
+             p[k11] * auxr * auxz;
+    }
+    return;
+  }
+  double w[4][4];
+#pragma unroll
+  for (int i = -1; i <= 2; ++i) {
+    double wr = 1.0;
+#pragma unroll
+    for (int m = -1; m <= 2; ++m) {
+      if (m == i) continue;
+      wr = wr * (auxr - static_cast<double>(m)) / static_cast<double>(i - m);
+    }
+#pragma unroll
+    for (int j = -1; j <= 2; ++j) {
+      double aux = wr;
+#pragma unroll
+      for (int n = -1; n <= 2; ++n) {
+        if (n == j) continue;
+        aux = aux * (auxz - static_cast<double>(n)) / static_cast<double>(j - n);
+      }
+      w[i + 1][j + 1] = aux;
+    }
+  }
+  // IDX(G, i, j) = row(i) + col(j)
+  long row[4];
+  int col[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    row[q] = static_cast<long>(ci(i0 - 1 + q) + g - 1) * G.ld;
+    col[q] = cj(j0 - 1 + q) + g - 1;
+  }
+#pragma unroll
+  for (int f = 0; f < AF_N; ++f) {
+    const double* __restrict__ p = L.p[f];
+    double out = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out = out + w[i][j] * p[row[i] + col[j]];
+    v[f] = out;
+  }
+}
+
+// ------------------------------------------------------------ ah_source
+// h'' (sdh) and the area element (sarea) at (h, h', theta) for a horizon
+// centred at zcenter (:1983-2752).  Returns the error flag.
+__device__ int ah_source(const AhCtx& c, double rad, double dh, double th, double zcenter, double& sdh,
+                         double& sarea) {
+  const AhLevel& L0 = c.L[0];
+  const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
+  const double smallpi = M_PI;
+  sdh = 0.0;
+  if (rad < 0.0) return 1;                                                       // :2140-2145
+  const double rc = rad * std::sin(th);                                          // :2151-2159
+  const double zc = rad * std::cos(th);
+  const double r0 = rc;
+  double z0 = zc + zcenter;
+  if (c.eqsym && z0 < 0.0) z0 = -z0;
+  // a non-finite position (never produced by a finite integration) is refused
+  // before it could index the grid
+  if (!(std::isfinite(r0) && std::isfinite(z0))) return 1;
+  if (r0 > c.ahrmax * L0.rmax || std::fabs(z0) > c.ahrmax * L0.zmax) return 1;  // :2163-2168
+  // :2172-2178 (the z test against zmax subtracts drl, as the reference does)
+  if (r0 < L0.rmin + L0.G.dr || r0 > L0.rmax - L0.G.dr || z0 < L0.zmin + L0.G.dz || z0 > L0.zmax - L0.G.dr) return 1;
+  if (z0 * zcenter < 0.0 && c.method == OA_AH_SHOOT) return 1;                  // :2185-2190
+
+  int lev = 0;                                                                   // :2199-2213
+  for (int ll = 1; ll < c.nlev; ++ll) {
+    const AhLevel& L = c.L[ll];
+    if (r0 > L.rmin + L.G.dr && r0 < L.rmax - L.G.dr && z0 > L.zmin + L.G.dz && z0 < L.zmax - L.G.dz) lev = ll;
+  }
+  double v[AF_N];
+  ah_interp(c.L[lev], c.interporder, r0, z0, v);
+
+  const double interp_psi4 = (v[AF_psi] * v[AF_psi]) * (v[AF_psi] * v[AF_psi]);
+  double Dphi[3];                                                                // :2247-2249
+  Dphi[0] = (rc * v[AF_Drphi] + zc * v[AF_Dzphi]) / rad;
+  Dphi[1] = (zc * v[AF_Drphi] - rc * v[AF_Dzphi]);
+  Dphi[2] = zero;
+
+  const double iA = v[AF_A], iDrA = v[AF_DrA], iDzA = v[AF_DzA];
+  const double iB = v[AF_B], iDrB = v[AF_DrB], iDzB = v[AF_DzB];
+  const double iC = v[AF_C], iDrC = v[AF_DrC], iDzC = v[AF_DzC];
+  const double iH = v[AF_H], iDrH = v[AF_DrH], iDzH = v[AF_DzH];
+  const double rc2 = rc * rc, zc2 = zc * zc, rad2 = rad * rad;
+
+  double gd[3][3], gu[3][3];                                                     // :2320-2340
+  gd[0][0] = (rc2 * iA + zc2 * iB + two * rc2 * zc * iC) / rad2;
+  gd[1][1] = (zc2 * iA + rc2 * iB - two * rc2 * zc * iC);
+  gd[0][1] = rc * (zc * (iA - iB) + (zc2 - rc2) * iC) / rad;
+  gd[2][2] = rc2 * iH;
+  gd[0][2] = zero;
+  gd[1][2] = zero;
+  gd[1][0] = gd[0][1];
+  gd[2][0] = gd[0][2];
+  gd[2][1] = gd[1][2];
+  if (th != zero && th != smallpi) {                                             // :2345-2378
+    const double aux = gd[0][0] * gd[1][1] - gd[0][1] * gd[0][1];
+    gu[0][0] = gd[1][1] / aux;
+    gu[1][1] = gd[0][0] / aux;
+    gu[2][2] = one / gd[2][2];
+    gu[0][1] = -gd[0][1] / aux;
+  } else {
+    gu[0][0] = one / gd[0][0];
+    gu[1][1] = one / gd[1][1];
+    gu[2][2] = zero;
+    gu[0][1] = zero;
+  }
+  gu[0][2] = zero;
+  gu[1][2] = zero;
+  gu[1][0] = gu[0][1];
+  gu[2][0] = gu[0][2];
+  gu[2][1] = gu[1][2];
+
+  double Dg[3][3][3];                                                            // :2388-2472
+  double diffr, diffz;
+  diffr = (two * rc * iA + rc2 * iDrA + zc2 * iDrB + two * rc * zc * (two * iC + rc * iDrC) - two * rc * gd[0][0]) / rad2;
+  diffz = (rc2 * iDzA + two * zc * iB + zc2 * iDzB + two * rc2 * (iC + zc * iDzC) - two * zc * gd[0][0]) / rad2;
+  Dg[0][0][0] = (rc * diffr + zc * diffz) / rad;
+  Dg[1][0][0] = (zc * diffr - rc * diffz);
+  diffr = (two * rc * iB + rc2 * iDrB + zc2 * iDrA - two * rc * zc * (two * iC + rc * iDrC));
+  diffz = (rc2 * iDzB + two * zc * iA + zc2 * iDzA - two * rc2 * (iC + zc * iDzC));
+  Dg[0][1][1] = (rc * diffr + zc * diffz) / rad;
+  Dg[1][1][1] = (zc * diffr - rc * diffz);
+  {
+    const double q = rc / rad;
+    diffr = ((one - q * q) * (zc * (iA - iB) + (zc2 - rc2) * iC) +
+             rc * (zc * (iDrA - iDrB) + (zc2 - rc2) * iDrC - two * rc * iC)) / rad;
+    diffz = q * (-(zc / rad2) * (zc * (iA - iB) + (zc2 - rc2) * iC) + (iA - iB) + zc * (iDzA - iDzB) +
+                 (zc2 - rc2) * iDzC + two * zc * iC);
+  }
+  Dg[0][0][1] = (rc * diffr + zc * diffz) / rad;
+  Dg[1][0][1] = (zc * diffr - rc * diffz);
+  diffr = rc * (two * iH + rc * iDrH);
+  diffz = rc2 * iDzH;
+  Dg[0][2][2] = (rc * diffr + zc * diffz) / rad;
+  Dg[1][2][2] = (zc * diffr - rc * diffz);
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    Dg[d][0][2] = zero;
+    Dg[d][1][2] = zero;
+    Dg[d][1][0] = Dg[d][0][1];
+    Dg[d][2][0] = Dg[d][0][2];
+    Dg[d][2][1] = Dg[d][1][2];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Dg[2][i][j] = zero;
+
+  double Kd[3][3];                                                               // :2510-2530
+  Kd[0][0] = (rc2 * v[AF_KA] + zc2 * v[AF_KB] + two * rc2 * zc * v[AF_KC]) / rad2;
+  Kd[1][1] = (zc2 * v[AF_KA] + rc2 * v[AF_KB] - two * rc2 * zc * v[AF_KC]);
+  Kd[0][1] = rc * (zc * (v[AF_KA] - v[AF_KB]) + (zc2 - rc2) * v[AF_KC]) / rad;
+  Kd[2][2] = rc2 * v[AF_KH];
+  Kd[0][2] = zero;
+  Kd[1][2] = zero;
+  Kd[1][0] = Kd[0][1];
+  Kd[2][0] = Kd[0][2];
+  Kd[2][1] = Kd[1][2];
+
+  double DF[3];                                                                  // :2537-2552
+#pragma unroll
+  for (int i = 0; i < 3; ++i) DF[i] = (gu[0][i] - gu[1][i] * dh) / interp_psi4;
+  const double u2 = (gu[0][0] - two * gu[0][1] * dh + gu[1][1] * dh * dh) / interp_psi4;
+  const double u = std::sqrt(std::fabs(u2));
+  const double denom = (gu[0][0] * gu[1][1] - gu[0][1] * gu[0][1]) / (interp_psi4 * interp_psi4);
+
+  double Gam[2][3][3];                                                           // :2575-2608
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double gm = zero;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) gm = gm + half * gu[k][m] * (Dg[i][j][m] + Dg[j][i][m] - Dg[m][i][j]);
+        const double dki = (k == i) ? one : zero, dkj = (k == j) ? one : zero;
+        Gam[k][i][j] = gm + two * (dki * Dphi[j] + dkj * Dphi[i] - gd[i][j] * (gu[k][0] * Dphi[0] + gu[k][1] * Dphi[1]));
+      }
+
+  double s = zero;                                                               // :2615-2650
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s = s + u2 * gu[i][j] * (Gam[0][i][j] - Gam[1][i][j] * dh) / interp_psi4;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s = s - DF[i] * DF[j] * (Gam[0][i][j] - Gam[1][i][j] * dh);
+  s = s + u * u * u * v[AF_trK];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s = s - u * DF[i] * DF[j] * Kd[i][j];
+  s = -s / denom;
+  if (std::fabs(th) < 1.0e-5 || std::fabs(th - smallpi) < 1.0e-5) s = zero;    // :2663-2667
+  if (s != s || std::fabs(s) > 1.0e50) {                                         // :2671-2676
+    sdh = 1.0e10;
+    return 1;
+  }
+  sdh = s;
+  sarea = two * smallpi * interp_psi4 * std::sqrt(std::fabs(gd[2][2] * (dh * dh * gd[0][0] + two * dh * gd[0][1] + gd[1][1])));
+  return 0;
+}
+
+// ------------------------------------------------------------ one shot
+// ahshoot :478-706: adaptive RK4 with step doubling from the axis point
+// zinit.  The reference's "local extrapolation" (:672-677) writes the slot
+// after the accepted one, which the next step overwrites before reading, so
+// it has no effect and is not restated.
+__device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, double* th_out, double* rad_out) {
+  const double zero = 0.0, half = 0.5, two = 2.0;
+  const double smallpi = M_PI;
+  double rad = std::fabs(zinit - zcenter);
+  double drad = 0.0, area = 0.0;
+  double dth = smallpi / 100.0;
+  double theta;
+  if (zcenter >= zero) {
+    theta = 1.0e-3;
+  } else {
+    dth = -dth;
+    theta = smallpi - 1.0e-3;
+  }
+  double thetafinal;
+  if (c.eqsym && zcenter == zero) thetafinal = half * smallpi;
+  else if (zcenter >= zero) thetafinal = smallpi - 1.0e-3;
+  else thetafinal = 1.0e-3;
+  if (th_out) {
+    th_out[0] = theta;
+    rad_out[0] = rad;
+  }
+  const double sixth = 1.0 / 6.0;
+  int i = 0, err = 0;
+  double sdh = 0.0, sarea = 0.0;
+  for (int steps = 0; std::fabs(theta - thetafinal) > 1.0e-3; ++steps) {
+    if (steps >= kAhMaxSteps) {
+      err = 1;
+      break;
+    }
+    if (zcenter >= zero) {
+      if (theta + dth > thetafinal) dth = thetafinal - theta;
+    } else {
+      if (theta + dth < thetafinal) dth = thetafinal - theta;
+    }
+    const double dth2 = half * dth;
+    double k_1, k_2, k_3, k_4, l_1, l_2, l_3, l_4, m_1, m_2, m_3, m_4;
+    if ((err = ah_source(c, rad, drad, theta, zcenter, sdh, sarea))) break;
+    k_1 = dth * drad;
+    l_1 = dth * sdh;
+    m_1 = std::fabs(dth2) * sarea;
+    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth, zcenter, sdh, sarea))) break;
+    k_2 = dth * (drad + 0.5 * l_1);
+    l_2 = dth * sdh;
+    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth, zcenter, sdh, sarea))) break;
+    k_3 = dth * (drad + 0.5 * l_2);
+    l_3 = dth * sdh;
+    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth, zcenter, sdh, sarea))) break;
+    k_4 = dth * (drad + l_3);
+    l_4 = dth * sdh;
+    const double drad_aux = drad + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
+
+    k_1 = half * k_1;
+    l_1 = half * l_1;
+    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth2, zcenter, sdh, sarea))) break;
+    k_2 = dth2 * (drad + 0.5 * l_1);
+    l_2 = dth2 * sdh;
+    m_2 = std::fabs(dth2) * sarea;
+    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth2, zcenter, sdh, sarea))) break;
+    k_3 = dth2 * (drad + 0.5 * l_2);
+    l_3 = dth2 * sdh;
+    m_3 = std::fabs(dth2) * sarea;
+    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth2, zcenter, sdh, sarea))) break;
+    k_4 = dth2 * (drad + l_3);
+    l_4 = dth2 * sdh;
+    m_4 = std::fabs(dth2) * sarea;
+    const double rad1 = rad + sixth * (k_1 + 2.0 * k_2 + 2.0 * k_3 + k_4);
+    const double drad1 = drad + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
+    const double area1 = area + sixth * (m_1 + 2.0 * m_2 + 2.0 * m_3 + m_4);
+
+    if ((err = ah_source(c, rad1, drad1, theta + dth, zcenter, sdh, sarea))) break;
+    k_1 = dth2 * drad1;
+    l_1 = dth2 * sdh;
+    m_1 = std::fabs(dth2) * sarea;
+    if ((err = ah_source(c, rad1 + 0.5 * k_1, drad1 + 0.5 * l_1, theta + 1.5 * dth2, zcenter, sdh, sarea))) break;
+    k_2 = dth2 * (drad1 + 0.5 * l_1);
+    l_2 = dth2 * sdh;
+    m_2 = std::fabs(dth2) * sarea;
+    if ((err = ah_source(c, rad1 + 0.5 * k_2, drad1 + 0.5 * l_2, theta + 1.5 * dth2, zcenter, sdh, sarea))) break;
+    k_3 = dth2 * (drad1 + 0.5 * l_2);
+    l_3 = dth2 * sdh;
+    m_3 = std::fabs(dth2) * sarea;
+    if ((err = ah_source(c, rad1 + k_3, drad1 + l_3, theta + dth, zcenter, sdh, sarea))) break;
+    k_4 = dth2 * (drad1 + l_3);
+    l_4 = dth2 * sdh;
+    m_4 = std::fabs(dth2) * sarea;
+    const double rad2 = rad1 + sixth * (k_1 + 2.0 * k_2 + 2.0 * k_3 + k_4);
+    const double drad2 = drad1 + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
+    const double area2 = area1 + sixth * (m_1 + 2.0 * m_2 + 2.0 * m_3 + m_4);
+
+    const double drad_error = std::fabs(drad2 - drad_aux);                       // :652-700
+    const double eps = 1.0e-4 * (std::fabs(drad2) + std::fabs(drad_aux));
+    if (drad_error > std::fmax(eps, 1.0e-8)) {
+      dth = half * dth;
+    } else {
+      theta = theta + dth;
+      rad = rad2;
+      drad = drad2;
+      area = area2;
+      i = i + 1;
+      if (th_out) {
+        th_out[i] = theta;
+        rad_out[i] = rad;
+      }
+      if (i >= kAhNthetaMax) {
+        err = 1;
+        break;
+      }
+      if (drad_error < 0.1 * eps) dth = two * dth;
+    }
+  }
+  ShotOut o;
+  o.dradlast = drad;
+  o.area = area;
+  o.err = err;
+  o.ntheta = i;
+  return o;
+}
+
+__global__ void __launch_bounds__(kShotBlock) k_ah_shoot(const ShotIn* __restrict__ in, ShotOut* __restrict__ out,
+                                                         int n) {
+  const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  const ShotIn s = in[t];
+  out[t] = ah_integrate(*s.ctx, s.zinit, s.zcenter, s.theta, s.rad);
+}
+
+// ================================================================ host side
+int hip_rc(hipError_t e, const char* what) {
+  if (e == hipSuccess) return OA_OK;
+  oa_set_error(std::string("oa_ahfind: ") + what + ": " + hipGetErrorString(e));
+  return OA_ERR_HIP;
+}
+#define AH_HIP(call)                             \
+  do {                                           \
+    int rc_ = hip_rc((call), #call);             \
+    if (rc_ != OA_OK) return rc_;                \
+  } while (0)
+#define AH_RC(call)                              \
+  do {                                           \
+    int rc_ = (call);                            \
+    if (rc_ != OA_OK) return rc_;                \
+  } while (0)
+
+struct ShotRes {
+  double dradlast, area;
+  int err, ntheta;
+};
+
+uint64_t key_of(double z) {
+  uint64_t k;
+  std::memcpy(&k, &z, sizeof k);
+  return k;
+}
+
+}  // namespace
+
+// Per-handle state of the finder: derived planes, the device context, the
+// shot tables, ahfound(k) (kept between searches as the reference keeps it)
+// and the last integration of each horizon.
+struct AhState {
+  std::vector<double*> planes;            // AP_N planes per level
+  AhCtx hctx{};
+  AhCtx* dctx = nullptr;
+  ShotIn* h_in = nullptr;                 // pinned
+  ShotOut* h_out = nullptr;               // pinned
+  ShotIn* d_in = nullptr;
+  ShotOut* d_out = nullptr;
+  int cap = 0;
+  double* d_theta = nullptr;              // one recorded integration
+  double* d_rad = nullptr;
+  int found[3] = {0, 0, 0};
+  std::vector<double> theta[3], rad[3];
+};
+
+namespace {
+
+// finest level holding the axis point (0, zinit) one cell inside (:462-476)
+double dz_on_axis(const AhCtx& c, double zinit) {
+  int level = 0;
+  for (int ll = 1; ll < c.nlev; ++ll) {
+    const AhLevel& L = c.L[ll];
+    if (0.0 > L.rmin + L.G.dr && 0.0 < L.rmax - L.G.dr && zinit > L.zmin + L.G.dz && zinit < L.zmax - L.G.dz)
+      level = ll;
+  }
+  return c.L[level].G.dz;
+}
+
+int ensure_capacity(AhState& a, int n) {
+  if (n <= a.cap) return OA_OK;
+  int cap = a.cap ? a.cap : 256;
+  while (cap < n) cap *= 2;
+  if (a.h_in) (void)hipHostFree(a.h_in);
+  if (a.h_out) (void)hipHostFree(a.h_out);
+  if (a.d_in) (void)hipFree(a.d_in);
+  if (a.d_out) (void)hipFree(a.d_out);
+  a.h_in = nullptr;
+  a.h_out = nullptr;
+  a.d_in = nullptr;
+  a.d_out = nullptr;
+  a.cap = 0;
+  AH_HIP(hipHostMalloc(reinterpret_cast<void**>(&a.h_in), sizeof(ShotIn) * cap, hipHostMallocDefault));
+  AH_HIP(hipHostMalloc(reinterpret_cast<void**>(&a.h_out), sizeof(ShotOut) * cap, hipHostMallocDefault));
+  AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_in), sizeof(ShotIn) * cap));
+  AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_out), sizeof(ShotOut) * cap));
+  a.cap = cap;
+  return OA_OK;
+}
+
+// Integrate the shots zs (all centred at zcenter) in one launch.
+int run_shots(oa_handle* h, AhState& a, const std::vector<double>& zs, double zcenter, bool record,
+              std::vector<ShotRes>& res) {
+  const int n = static_cast<int>(zs.size());
+  res.assign(static_cast<size_t>(n), ShotRes{0.0, 0.0, 0, 0});
+  if (n == 0) return OA_OK;
+  AH_RC(ensure_capacity(a, n));
+  for (int t = 0; t < n; ++t) {
+    a.h_in[t].ctx = a.dctx;
+    a.h_in[t].zinit = zs[static_cast<size_t>(t)];
+    a.h_in[t].zcenter = zcenter;
+    a.h_in[t].theta = record ? a.d_theta : nullptr;
+    a.h_in[t].rad = record ? a.d_rad : nullptr;
+  }
+  hipStream_t st = h->stream;
+  AH_HIP(hipMemcpyAsync(a.d_in, a.h_in, sizeof(ShotIn) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_ah_shoot, dim3((n + kShotBlock - 1) / kShotBlock), dim3(kShotBlock), 0, st, a.d_in, a.d_out, n);
+  AH_HIP(hipGetLastError());
+  AH_HIP(hipMemcpyAsync(a.h_out, a.d_out, sizeof(ShotOut) * n, hipMemcpyDeviceToHost, st));
+  AH_HIP(hipStreamSynchronize(st));
+  for (int t = 0; t < n; ++t)
+    res[static_cast<size_t>(t)] = ShotRes{a.h_out[t].dradlast, a.h_out[t].area, a.h_out[t].err, a.h_out[t].ntheta};
+  return OA_OK;
+}
+
+// One search (ahshoot :377-855) as a walk over a table of shot results.
+struct Search {
+  const AhCtx* c;
+  double zcenter, eps;
+  int found_in;
+  std::map<uint64_t, ShotRes> table;
+  // outcome of a complete walk
+  int found = 0, errorflag = 0, shots = 0;
+  double zinit = 0.0, dradlast = 0.0, last_zinit = 0.0;
+  ShotRes last{0.0, 0.0, 0, 0};
+};
+
+// Walk the reference's control flow from the start.  Returns true when the
+// search is complete; otherwise `want` holds the shots to integrate next
+// (the missing one first, then its speculative continuation).
+bool walk(Search& s, std::vector<double>& want) {
+  const AhCtx& c = *s.c;
+  const AhLevel& L0 = c.L[0];
+  const double zero = 0.0, half = 0.5, two = 2.0;
+  const double zcenter = s.zcenter;
+  const bool up = zcenter >= zero;
+  want.clear();
+
+  double zinit_old = zero, zinit;
+  // ahzinit is resolved by the caller into s.zinit's start (see ah_search)
+  zinit = s.zinit;
+  double dzlocal = L0.G.dz, dzz = half * dzlocal;
+  double dradlast = 0.0, dradlast_old = 0.0, drada = 0.0, za, zb;
+  int errorflag = 0, shots = 0, found = s.found_in;
+  ShotRes last{0.0, 0.0, 0, 0};
+  double last_zinit = zinit;
+
+  for (;;) {                                                          // 50 continue
+    bool bracket = false, firstiter = true;
+    errorflag = 0;
+    za = 1.0e10;
+    zb = -1.0e10;
+    while (std::fabs(zinit - zcenter) > 5.0 * dzlocal && std::fabs(zinit_old - zinit) > s.eps) {
+      dzlocal = dz_on_axis(c, zinit);
+      dzz = half * dzlocal;
+      dradlast_old = firstiter ? 1.0e10 : dradlast;
+      const auto it = s.table.find(key_of(zinit));
+      if (it == s.table.end()) {
+        // speculate from here
+        if (!bracket) {
+          // the march: zinit, zinit -/+ dzz(zinit), ... while the loop
+          // condition would still hold
+          double z = zinit, zold = zinit_old, dzl = dzlocal;
+          for (int q = 0; q < kAhScanChunk; ++q) {
+            if (q > 0 && !(std::fabs(z - zcenter) > 5.0 * dzl && std::fabs(zold - z) > s.eps)) break;
+            want.push_back(z);
+            dzl = dz_on_axis(c, z);
+            zold = z;
+            z = up ? z - half * dzl : z + half * dzl;
+          }
+        } else {
+          // the bisection: every node of the next kAhDepth steps; node
+          // (a, b) shoots half*(a+b) and continues in (a, m) or (m, b)
+          std::vector<std::pair<double, double>> level{{za, zb}};
+          for (int d = 0; d < kAhDepth; ++d) {
+            std::vector<std::pair<double, double>> next;
+            for (const auto& ab : level) {
+              const double m = half * (ab.first + ab.second);
+              want.push_back(m);
+              next.push_back({ab.first, m});
+              next.push_back({m, ab.second});
+            }
+            level.swap(next);
+          }
+        }
+        return false;
+      }
+      last = it->second;
+      last_zinit = zinit;
+      ++shots;
+      errorflag = last.err;
+      dradlast = last.dradlast;
+      if (firstiter || (!bracket && dradlast * dradlast_old >= zero)) {  // :739-792
+        firstiter = false;
+        zinit_old = zinit;
+        zinit = up ? zinit - dzz : zinit + dzz;
+      } else if (!bracket) {
+        bracket = true;
+        za = zinit_old;
+        zb = zinit;
+        drada = dradlast_old;
+        zinit_old = zinit;
+        zinit = half * (za + zb);
+      } else {
+        if (drada * dradlast <= zero) {
+          zb = zinit;
+        } else {
+          za = zinit;
+          drada = dradlast;
+        }
+        zinit_old = zinit;
+        zinit = half * (za + zb);
+      }
+    }
+    if (std::fabs(zinit - zinit_old) <= s.eps) {                       // :797-855
+      if (!errorflag && std::fabs(dradlast) < 1.0e-1) {
+        found = 1;
+      } else if (std::fabs(zinit - zcenter) > dzlocal) {
+        zinit = up ? zinit - two * dzz : zinit + two * dzz;
+        continue;
+      }
+    } else {
+      found = 0;
+    }
+    break;
+  }
+  s.found = found;
+  s.errorflag = errorflag;
+  s.shots = shots;
+  s.zinit = zinit;
+  s.dradlast = dradlast;
+  s.last = last;
+  s.last_zinit = last_zinit;
+  return true;
+}
+
+int build_fields(oa_handle* h, AhState& a) {
+  const int nlev = h->nlev;
+  if (static_cast<int>(a.planes.size()) != nlev) {
+    a.planes.assign(static_cast<size_t>(nlev), nullptr);
+    for (int l = 0; l < nlev; ++l) {
+      const size_t bytes = sizeof(double) * AP_N * static_cast<size_t>(h->L[l].G.plane);
+      if (hipMalloc(reinterpret_cast<void**>(&a.planes[static_cast<size_t>(l)]), bytes) != hipSuccess) {
+        oa_set_error("oa_ahfind: device allocation of the horizon planes failed");
+        return OA_ERR_NOMEM;
+      }
+      AH_HIP(hipMemsetAsync(a.planes[static_cast<size_t>(l)], 0, bytes, h->stream));
+    }
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.dctx), sizeof(AhCtx)));
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_theta), sizeof(double) * (kAhNthetaMax + 1)));
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_rad), sizeof(double) * (kAhNthetaMax + 1)));
+  }
+  const oa_params& p = h->p;
+  AhCtx& c = a.hctx;
+  std::memset(&c, 0, sizeof c);
+  c.nlev = nlev;
+  c.eqsym = p.eqsym;
+  c.interporder = p.interporder;
+  c.method = p.ahmethod;
+  c.ahrmax = (p.ahrmax > 0.9) ? 0.9 : p.ahrmax;
+  for (int l = 0; l < nlev; ++l) {
+    Level& L = h->L[l];
+    const KGrid& G = L.G;
+    const double* y = L.slab[L.in_step ? L.last_out : L.cur];
+    double* out = a.planes[static_cast<size_t>(l)];
+    const dim3 blk(64, 4, 1);
+    const dim3 grd((G.Nz - G.jlo + 1 + 63) / 64, (G.Nr + 3) / 4, 1);
+    const int from_chi = L.phi_valid ? 0 : 1;
+    if (L.aux_phi) hipLaunchKernelGGL((k_ah_fields<true>), grd, blk, 0, h->stream, G, y, from_chi, out);
+    else hipLaunchKernelGGL((k_ah_fields<false>), grd, blk, 0, h->stream, G, y, from_chi, out);
+    AH_HIP(hipGetLastError());
+    AhLevel& A = c.L[l];
+    A.G = G;
+    const int g = G.g;
+    A.rmin = (static_cast<double>(1 - g) - 0.5) * G.dr;
+    A.rmax = (static_cast<double>(G.Nr) - 0.5) * G.dr;
+    A.zmin = (static_cast<double>(1 - g) - G.zoff) * G.dz;
+    A.zmax = (static_cast<double>(G.Nz) - G.zoff) * G.dz;
+    const long pl = G.plane;
+    const int derived[AF_N] = {AP_psi, AP_Drphi, AP_Dzphi, -S_A, AP_DrA, AP_DzA, -S_B, AP_DrB, AP_DzB,
+                               -S_C, AP_DrC, AP_DzC, -S_H, AP_DrH, AP_DzH, -S_trK, AP_KA, AP_KB, AP_KC, AP_KH};
+    for (int f = 0; f < AF_N; ++f) {
+      const bool state = (f == AF_A || f == AF_B || f == AF_C || f == AF_H || f == AF_trK);
+      A.p[f] = state ? y + static_cast<long>(-derived[f]) * pl : out + static_cast<long>(derived[f]) * pl;
+    }
+  }
+  AH_HIP(hipMemcpyAsync(a.dctx, &a.hctx, sizeof(AhCtx), hipMemcpyHostToDevice, h->stream));
+  return OA_OK;
+}
+
+}  // namespace
+
+void ah_release(oa_handle* h) {
+  AhState* a = h->ah;
+  if (!a) return;
+  for (double* p : a->planes)
+    if (p) (void)hipFree(p);
+  if (a->dctx) (void)hipFree(a->dctx);
+  if (a->d_theta) (void)hipFree(a->d_theta);
+  if (a->d_rad) (void)hipFree(a->d_rad);
+  if (a->d_in) (void)hipFree(a->d_in);
+  if (a->d_out) (void)hipFree(a->d_out);
+  if (a->h_in) (void)hipHostFree(a->h_in);
+  if (a->h_out) (void)hipHostFree(a->h_out);
+  delete a;
+  h->ah = nullptr;
+}
+
+}  // namespace oax
+
+using namespace oax;
+
+extern "C" int oa_ahfind(oa_handle* h, int k, oa_horizon* out) {
+  if (!h || !out) {
+    oa_set_error("oa_ahfind: null argument");
+    return OA_ERR_ARG;
+  }
+  const oa_params& p = h->p;
+  if (p.N_ah > 3 || p.N_ah < 1 || k < 0 || k >= p.N_ah) {                  // :48-57
+    oa_set_error("oa_ahfind: horizon index out of range (N_ah <= 3)");
+    return OA_ERR_ARG;
+  }
+  if (p.eqsym && p.N_ah == 3) {                                              // :62-71
+    oa_set_error("oa_ahfind: for equatorial symmetry we can only look for 2 horizons");
+    return OA_ERR_PARAM;
+  }
+  if (p.z_ah1 < 0.0 || p.z_ah2 > 0.0) {                                       // :76-95
+    oa_set_error("oa_ahfind: horizon 1 must be above the equator and horizon 2 below");
+    return OA_ERR_PARAM;
+  }
+  if (p.ahmove || p.ahmethod != OA_AH_SHOOT) {
+    oa_set_error("oa_ahfind: only ahmethod = shoot with ahmove = .false. is supported");
+    return OA_ERR_UNSUPPORTED;
+  }
+  if (h->nlev > kAhMaxLevels) {
+    oa_set_error("oa_ahfind: too many levels");
+    return OA_ERR_UNSUPPORTED;
+  }
+  for (const Level& L : h->L)
+    if (L.dirty) {
+      oa_set_error("oa_ahfind: state not projected (call oa_auxiliary / oa_initial_aux)");
+      return OA_ERR_STATE;
+    }
+  if (!h->ah) h->ah = new AhState();
+  AhState& a = *h->ah;
+  AH_RC(build_fields(h, a));
+
+  double zah[3] = {0.0, 0.0, 0.0};                                           // :104-111
+  if (p.N_ah > 1) {
+    zah[1] = p.z_ah1;
+    zah[2] = p.z_ah2;
+  }
+  const double zcenter = zah[k];
+  const AhLevel& L0 = a.hctx.L[0];
+  Search s;
+  s.c = &a.hctx;
+  s.zcenter = zcenter;
+  s.eps = p.ahepsilon_shoot;
+  s.found_in = a.found[k];
+  if (p.ahzinit == 0.0) {                                                    // :379-405
+    s.zinit = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
+  } else if (p.ahzinit > 0.0 && p.ahzinit <= 0.9 * L0.zmax) {
+    s.zinit = (zcenter >= 0.0) ? p.ahzinit : -p.ahzinit;
+  } else {
+    s.zinit = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
+  }
+  const double zstart = s.zinit;
+  std::vector<double> want;
+  std::vector<ShotRes> res;
+  for (;;) {
+    s.zinit = zstart;
+    if (walk(s, want)) break;
+    AH_RC(run_shots(h, a, want, zcenter, false, res));
+    for (size_t q = 0; q < want.size(); ++q) s.table[key_of(want[q])] = res[q];
+  }
+  a.found[k] = s.found;
+  // the last integration, recorded (ah_radius_SH, :956-966)
+  if (s.shots > 0) {
+    AH_RC(run_shots(h, a, std::vector<double>{s.last_zinit}, zcenter, true, res));
+    const int n = res[0].ntheta + 1;
+    a.theta[k].resize(static_cast<size_t>(n));
+    a.rad[k].resize(static_cast<size_t>(n));
+    AH_HIP(hipMemcpyAsync(a.theta[k].data(), a.d_theta, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    AH_HIP(hipMemcpyAsync(a.rad[k].data(), a.d_rad, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    AH_HIP(hipStreamSynchronize(h->stream));
+  } else {
+    a.theta[k].clear();
+    a.rad[k].clear();
+  }
+  if (!s.found)
+    for (double& v : a.rad[k]) v = 0.0;
+
+  const double smallpi = M_PI, half = 0.5, two = 2.0;
+  std::memset(out, 0, sizeof *out);
+  out->found = s.found;
+  out->ntheta = s.last.ntheta;
+  out->shots = s.shots;
+  out->errorflag = s.errorflag;
+  out->zcenter = zcenter;
+  out->zinit = s.zinit;
+  out->dradlast = s.dradlast;
+  double ah_area = 0.0;                                                      // :989-997
+  if (s.found) {
+    ah_area = s.last.area;
+    if (p.eqsym && zcenter == 0.0) ah_area = two * ah_area;
+  }
+  out->area = ah_area;
+  out->radius = half * std::sqrt(ah_area / smallpi);
+  out->mass = s.found ? half * out->radius : 0.0;                            // :1065-1076
+  return OA_OK;
+}
+
+extern "C" int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n) {
+  if (!h || !n || k < 0 || k > 2) {
+    oa_set_error("oa_horizon_shape: bad argument");
+    return OA_ERR_ARG;
+  }
+  if (!h->ah || h->ah->theta[k].empty()) {
+    oa_set_error("oa_horizon_shape: no search of this horizon yet (call oa_ahfind)");
+    return OA_ERR_STATE;
+  }
+  const AhState& a = *h->ah;
+  const int m = static_cast<int>(a.theta[k].size());
+  *n = m;
+  if (!theta || !rad || cap < m) {
+    oa_set_error("oa_horizon_shape: arrays too small (need *n entries)");
+    return OA_ERR_ARG;
+  }
+  std::memcpy(theta, a.theta[k].data(), sizeof(double) * m);
+  std::memcpy(rad, a.rad[k].data(), sizeof(double) * m);
+  return OA_OK;
+}

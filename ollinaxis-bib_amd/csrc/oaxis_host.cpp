@@ -462,6 +462,7 @@ extern "C" int oa_destroy(oa_handle* h) {
       if (L.fbtab[k]) (void)hipFree(L.fbtab[k]);
     if (L.cons) (void)hipFree(L.cons);
   }
+  ah_release(h);
   h->plan.release();
   if (h->norms_part) (void)hipFree(h->norms_part);
   if (h->ev_begin) (void)hipEventDestroy(h->ev_begin);

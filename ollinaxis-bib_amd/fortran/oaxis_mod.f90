@@ -27,7 +27,13 @@ module oaxis_mod
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
             oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule, &
-            oa_constraints
+            oa_constraints, oa_ahfind, oa_horizon_shape
+
+  ! result of one apparent-horizon search (oa_horizon in include/oaxis.h)
+  type, bind(C), public :: oa_horizon
+    integer(c_int) :: found, ntheta, shots, errorflag
+    real(c_double) :: zcenter, zinit, dradlast, area, radius, mass
+  end type oa_horizon
 
   ! storage for one oa_params block
   type :: oaxis_params
@@ -236,6 +242,21 @@ module oaxis_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int), value :: box, level
+    end function
+    ! horizon_finder (shooting) for horizon k on the current state
+    integer(c_int) function oa_ahfind(h, k, res) bind(C, name="oa_ahfind")
+      import :: c_int, c_ptr, oa_horizon
+      type(c_ptr), value :: h
+      integer(c_int), value :: k
+      type(oa_horizon) :: res
+    end function
+    ! (theta, h(theta)) of the last integration of horizon k (ah_radius_SH)
+    integer(c_int) function oa_horizon_shape(h, k, theta, rad, cap, n) bind(C, name="oa_horizon_shape")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: k, cap
+      real(c_double), dimension(*) :: theta, rad
+      integer(c_int) :: n
     end function
     ! 0D norms (max, min, nm1, nm2) of an evolved array, as save0Ddata forms them
     integer(c_int) function oa_norms0d(h, box, level, name, norms) bind(C, name="oa_norms0d")

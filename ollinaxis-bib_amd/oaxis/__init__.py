@@ -196,6 +196,26 @@ class Spacetime:
         mom_z, CDelta_r, CDelta_z can be downloaded or reduced by name."""
         _check(lib().oa_constraints(self._h, 0, level), "oa_constraints")
 
+    def ahfind(self, k=0, shape=False):
+        """horizon_finder (shooting) for horizon k on the current state:
+        dict(found, ntheta, shots, errorflag, zcenter, zinit, dradlast, area,
+        radius, mass); with shape=True also theta and rad of the last
+        integration (numpy arrays)."""
+        res = _abi.Horizon()
+        _check(lib().oa_ahfind(self._h, int(k), ctypes.byref(res)), "oa_ahfind")
+        out = {name: getattr(res, name) for name, _ in _abi.Horizon._fields_}
+        if shape:
+            import numpy as np
+            n = ctypes.c_int()
+            lib().oa_horizon_shape(self._h, int(k), None, None, 0, ctypes.byref(n))
+            th = np.zeros(max(n.value, 1))
+            rd = np.zeros(max(n.value, 1))
+            dp = ctypes.POINTER(ctypes.c_double)
+            _check(lib().oa_horizon_shape(self._h, int(k), th.ctypes.data_as(dp), rd.ctypes.data_as(dp),
+                                          int(th.size), ctypes.byref(n)), "oa_horizon_shape")
+            out["theta"], out["rad"] = th[:n.value], rd[:n.value]
+        return out
+
     def norms0d(self, name, level=0):
         """save0Ddata's norms of an evolved array: dict(max, min, nm1, nm2)."""
         out = (ctypes.c_double * 4)()

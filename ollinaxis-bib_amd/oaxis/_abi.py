@@ -16,22 +16,24 @@ LIBPATH = os.path.join(_HERE, "liboaxis.so")
 OA_OK = 0
 
 
-def _parse_params_struct(path=HEADER):
-    """Field list of ``oa_params`` read from the public header, so the ctypes
-    mirror cannot drift from the C definition."""
+def _parse_params_struct(path=HEADER, name="oa_params"):
+    """Field list of a plain struct (``oa_params``, ``oa_horizon``) read from
+    the public header, so the ctypes mirror cannot drift from the C
+    definition."""
     text = open(path).read()
-    body = text[text.index("typedef struct oa_params {"):text.index("} oa_params;")]
+    body = text[text.index("typedef struct %s {" % name):text.index("} %s;" % name)]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = []
     for line in body.split("\n")[1:]:
-        line = line.split("/*")[0].strip()
+        line = line.strip()
         if not line:
             continue
         m = re.match(r"(int|double)\s+(.*);", line)
         if not m:
-            raise RuntimeError("unexpected line in oa_params: %r" % line)
+            raise RuntimeError("unexpected line in %s: %r" % (name, line))
         ctype = ctypes.c_int if m.group(1) == "int" else ctypes.c_double
-        for name in m.group(2).split(","):
-            fields.append((name.strip(), ctype))
+        for field in m.group(2).split(","):
+            fields.append((field.strip(), ctype))
     return fields
 
 
@@ -58,6 +60,10 @@ def _parse_enums(path=HEADER):
 
 class Params(ctypes.Structure):
     _fields_ = _parse_params_struct()
+
+
+class Horizon(ctypes.Structure):
+    _fields_ = _parse_params_struct(name="oa_horizon")
 
 
 ENUMS = _parse_enums()
@@ -109,6 +115,8 @@ def load_library(path=LIBPATH):
         "oa_point_updates": (ctypes.c_int, [H, dp]),
         "oa_norms0d": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp]),
         "oa_constraints": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
+        "oa_ahfind": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(Horizon)]),
+        "oa_horizon_shape": (ctypes.c_int, [H, ctypes.c_int, dp, dp, ctypes.c_int, ip]),
         "oa_params_footprint": (ctypes.c_int, [P, dp]),
         "oa_batch_create": (ctypes.c_int, [ctypes.POINTER(H)]),
         "oa_batch_destroy": (ctypes.c_int, [H]),

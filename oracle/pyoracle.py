@@ -56,6 +56,9 @@ def lib():
             "or_run_restrict": (ctypes.c_int, [H, ctypes.c_int]),
             "or_run_update": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             "or_run_symmetries": (ctypes.c_int, [H, ctypes.c_int]),
+            "or_ahfind_flat": (ctypes.c_int, [H, ctypes.c_int, ip, dp, dp, dp]),
+            "or_ah_integrate": (ctypes.c_int, [H, ctypes.c_double, ctypes.c_double, dp]),
+            "or_ah_set_tolerance": (None, [ctypes.c_double, ctypes.c_double]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -63,6 +66,12 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def ah_tolerance(rel=1.0e-4, abs_=1.0e-8):
+    """Step-control tolerances of the oracle's horizon integration (the
+    reference's by default; process-wide)."""
+    lib().or_ah_set_tolerance(float(rel), float(abs_))
 
 
 def _dptr(a):
@@ -183,6 +192,38 @@ class Oracle:
 
     def run_restrict(self, level):
         lib().or_run_restrict(self._h, level)
+
+    AH_KEYS = ("found", "ntheta", "shots", "errorflag", "zcenter", "zinit", "dradlast", "area", "radius", "mass")
+
+    def ahfind(self, k=0, shape=False):
+        """horizon_finder for horizon k on the current state and auxiliary
+        arrays (every level).  ahfound(k) carries over between calls as in
+        the reference.  Returns a dict; with shape=True also theta and rad of
+        the last integration."""
+        if not hasattr(self, "_ahfound"):
+            self._ahfound = [ctypes.c_int(0) for _ in range(3)]
+        res = np.zeros(10)
+        th = np.zeros(50001)
+        rd = np.zeros(50001)
+        rc = lib().or_ahfind_flat(self._h, int(k), ctypes.byref(self._ahfound[k]), _dptr(res), _dptr(th), _dptr(rd))
+        if rc:
+            raise RuntimeError("oracle ahfind: error %d" % rc)
+        out = dict(zip(self.AH_KEYS, res.tolist()))
+        for key in ("found", "ntheta", "shots", "errorflag"):
+            out[key] = int(out[key])
+        if shape:
+            n = out["ntheta"] + 1
+            # ah_radius_SH holds h = 0 when no horizon was found (:962-966)
+            out["theta"] = th[:n].copy()
+            out["rad"] = rd[:n].copy() if out["found"] else np.zeros(n)
+        return out
+
+    def ah_integrate(self, zinit, zcenter=0.0):
+        """One integration of the horizon ODE from the axis point zinit:
+        (dradlast, area, errorflag, ntheta)."""
+        res = np.zeros(4)
+        lib().or_ah_integrate(self._h, float(zinit), float(zcenter), _dptr(res))
+        return res[0], res[1], int(res[2]), int(res[3])
 
     def coords(self, level=0):
         return self.get("r", level), self.get("z", level)

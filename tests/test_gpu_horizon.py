@@ -1,0 +1,146 @@
+"""GPU: the apparent-horizon finder (horizon_finder.f90, shooting method) on
+the device against the oracle, on the same states.
+
+The device walks the reference's sequence of shots and decisions, so the
+number of integrations, the axis position where the search stops, the
+found flag and the accepted steps of the last integration agree exactly;
+the area and the shape of the last integration agree to 1e-10 relative
+(the interpolated fields differ from the oracle's by rounding only).
+States: Schwarzschild on one and two levels (initial data and after
+coarse steps of 1+log slicing), equatorial symmetry, horizons centred off
+the origin, Schwarzschild through a non-trivial coordinate change,
+Painleve-Gullstrand (curvature-only horizon) and flat space (none).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from test_oracle_horizon import (disguised_schwarzschild, oracle_with, painleve_gullstrand,
+                                 schwarzschild_fields)
+
+pytestmark = pytest.mark.gpu
+
+EXACT = ("found", "shots", "errorflag", "ntheta", "zinit", "zcenter")
+TOL = 1e-10
+
+
+def device_from_oracle(oaxis, orc, p):
+    st = oaxis.Spacetime(p)
+    for lev in range(st.nlevels):
+        for name in oaxis.state_fields(p):
+            st.upload(name, orc.get(name, lev), lev)
+    return st
+
+
+def compare(dev, ref, label):
+    for key in EXACT:
+        assert dev[key] == ref[key], (label, key, dev[key], ref[key])
+    scale = max(abs(ref["area"]), 1.0)
+    assert abs(dev["area"] - ref["area"]) <= TOL * scale, (label, dev["area"], ref["area"])
+    assert abs(dev["mass"] - ref["mass"]) <= TOL * max(abs(ref["mass"]), 1.0), label
+    assert abs(dev["dradlast"] - ref["dradlast"]) <= 1e-8 * max(abs(ref["dradlast"]), 1.0), label
+    n = ref["ntheta"] + 1
+    assert dev["theta"].shape == (n,) and dev["rad"].shape == (n,)
+    assert np.abs(dev["theta"] - ref["theta"]).max() <= TOL, label
+    assert np.abs(dev["rad"] - ref["rad"]).max() <= TOL * max(np.abs(ref["rad"]).max(), 1.0), label
+
+
+def synthetic(oaxis, oracle, h, fields, k=0, extra="", eqsym=False):
+    orc = oracle_with(oaxis, oracle, h, fields, extra=extra, eqsym=eqsym)
+    p = orc._params
+    st = device_from_oracle(oaxis, orc, p)
+    st.initial_aux()
+    t0 = time.perf_counter()
+    dev = st.ahfind(k, shape=True)
+    t1 = time.perf_counter()
+    ref = orc.ahfind(k, shape=True)
+    t2 = time.perf_counter()
+    st.close()
+    orc.close()
+    print("horizon k=%d: found %d after %d shots, mass %.10f; device %.3f s, oracle %.3f s"
+          % (k, dev["found"], dev["shots"], dev["mass"], t1 - t0, t2 - t1))
+    return dev, ref
+
+
+def test_schwarzschild_matches_oracle(oaxis_mod, oracle_mod):
+    dev, ref = synthetic(oaxis_mod, oracle_mod, 0.05, schwarzschild_fields(2.0))
+    assert dev["found"] == 1 and abs(dev["mass"] - 2.0) < 2e-4
+    compare(dev, ref, "schwarzschild")
+
+
+def test_equatorial_symmetry_matches_oracle(oaxis_mod, oracle_mod):
+    dev, ref = synthetic(oaxis_mod, oracle_mod, 0.05, schwarzschild_fields(2.0), eqsym=True)
+    assert dev["found"] == 1
+    compare(dev, ref, "eqsym")
+
+
+@pytest.mark.parametrize("k,z0", [(1, 1.2), (2, -1.2)])
+def test_off_centre_horizon_matches_oracle(oaxis_mod, oracle_mod, k, z0):
+    extra = "N_ah = 3\nz_ah1 = 1.2\nz_ah2 = -1.2\n"
+    dev, ref = synthetic(oaxis_mod, oracle_mod, 0.05, schwarzschild_fields(1.0, z0), k=k, extra=extra)
+    assert dev["found"] == 1 and dev["zcenter"] == z0
+    compare(dev, ref, "off-centre %d" % k)
+
+
+def test_disguised_schwarzschild_matches_oracle(oaxis_mod, oracle_mod):
+    fields, zstar = disguised_schwarzschild(2)
+    dev, ref = synthetic(oaxis_mod, oracle_mod, 0.05, fields, extra="ahzinit = 1.3\n")
+    assert dev["found"] == 1 and abs(dev["zinit"] - zstar) < 1e-4
+    compare(dev, ref, "disguised")
+
+
+def test_painleve_gullstrand_matches_oracle(oaxis_mod, oracle_mod):
+    dev, ref = synthetic(oaxis_mod, oracle_mod, 0.1, painleve_gullstrand(0.5))
+    assert dev["found"] == 1 and abs(dev["mass"] - 0.5) < 1e-4
+    compare(dev, ref, "painleve-gullstrand")
+
+
+def test_flat_space_no_horizon(oaxis_mod, oracle_mod):
+    dev, ref = synthetic(oaxis_mod, oracle_mod, 0.1, {})
+    assert dev["found"] == 0 and dev["area"] == 0.0
+    compare(dev, ref, "flat")
+
+
+EVOLVED = """
+order = four
+dtfac = 0.25
+boundtype = radiative
+geodiss = 0.1
+idata = schwarzschild
+BH1mass = 2.0
+ilapse = psiminus2
+slicing = 1+log
+gauge_f = 2.0
+integrator = rk4
+eqsym = .false.
+dr0 = 0.1
+dz0 = 0.1
+Nrtotal = 32
+Nztotal = 64
+Nl0 = 1
+"""
+
+
+@pytest.mark.parametrize("steps", [0, 3])
+def test_evolved_two_levels_match_oracle(oaxis_mod, oracle_mod, steps):
+    p = oaxis_mod.params_from_text(EVOLVED)
+    orc = oracle_mod.Oracle(p)
+    orc.initial_data()
+    st = device_from_oracle(oaxis_mod, orc, p)
+    orc.initial_aux()
+    st.initial_aux()
+    for _ in range(steps):
+        orc.coarse_step()
+        st.coarse_step()
+    dev = st.ahfind(0, shape=True)
+    ref = orc.ahfind(0, shape=True)
+    print("evolved %d steps: found %d, %d shots, mass %.10f" % (steps, dev["found"], dev["shots"], dev["mass"]))
+    assert dev["found"] == 1
+    compare(dev, ref, "evolved %d" % steps)
+    # a second search on the same state repeats the first
+    again = st.ahfind(0, shape=True)
+    for key in EXACT + ("area",):
+        assert again[key] == dev[key]
+    st.close()
+    orc.close()
