@@ -326,6 +326,12 @@ int oa_batch_point_updates(const oa_batch* b, double* updates);
 /* Launches enqueued by the last oa_batch_coarse_step (one per kernel kind
  * and physics group) and the work items they covered. */
 int oa_batch_launch_stats(const oa_batch* b, long* launches, long* items);
+/* horizon_finder for horizon k[i] of members[i], i < n (every one a member
+ * of b), as oa_ahfind does for each alone, with the same results: the
+ * searches advance in lockstep and each round of shots of all of them is
+ * one launch.  For the serving loop, which checks many spacetimes for
+ * horizon formation at once. */
+int oa_batch_ahfind(oa_batch* b, int n, oa_handle* const* members, const int* k, oa_horizon* out);
 /* HIP-event timing of every RHS launch during oa_batch_evolve (off by
  * default): total kernel time (ms), launches, and grid points (Nr x Nz of
  * each item) those launches advanced by one substep. */

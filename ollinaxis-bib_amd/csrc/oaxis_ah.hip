@@ -12,10 +12,13 @@
 // interpolate 20 metric quantities, so one shot is a long dependent chain:
 // this file runs one shot per lane and many shots per launch.
 //
-//  1. k_ah_fields: one streaming pass per level forms the derived planes the
-//     ODE interpolates (psi, the conformal-factor and metric first
-//     derivatives, the physical extrinsic curvature), with their parity
-//     images in the ghost zones, as the auxiliary arrays hold them.
+//  1. k_ah_fields: one pass per level forms the 20 quantities the ODE
+//     interpolates (psi, the conformal-factor and metric first derivatives,
+//     A, B, C, H, trK, the physical extrinsic curvature), with their parity
+//     images in the ghost zones as the auxiliary arrays hold them, stored
+//     point by point (20 consecutive doubles per grid point): one bicubic
+//     stencil then reads 16 runs of 160 bytes instead of 320 scattered
+//     values.
 //  2. k_ah_shoot: one lane integrates one shot (zinit) to the end.
 //  3. The host walks the reference's control flow (march, bracket, bisection,
 //     restart; ahshoot :434-855) over a table of shot results.  When the walk
@@ -55,7 +58,7 @@ constexpr int kAhNthetaMax = 50000;                 // Nthetamax (:350)
 // bound is reached only by a shot the reference would integrate for hours.
 constexpr int kAhMaxSteps = 4 * kAhNthetaMax;
 constexpr int kAhScanChunk = 1024;                  // march positions per launch
-constexpr int kAhDepth = 8;                         // bisection steps per launch
+constexpr int kAhDepth = 11;                        // bisection steps per launch
 constexpr int kShotBlock = 64;
 
 // interpolated quantities, in ah_source's order
@@ -64,16 +67,12 @@ enum AhField {
   AF_A, AF_DrA, AF_DzA, AF_B, AF_DrB, AF_DzB, AF_C, AF_DrC, AF_DzC, AF_H, AF_DrH, AF_DzH,
   AF_trK, AF_KA, AF_KB, AF_KC, AF_KH, AF_N
 };
-// derived planes k_ah_fields writes (the other AF_* are state planes)
-enum AhPlane {
-  AP_psi = 0, AP_Drphi, AP_Dzphi, AP_DrA, AP_DzA, AP_DrB, AP_DzB, AP_DrC, AP_DzC, AP_DrH, AP_DzH,
-  AP_KA, AP_KB, AP_KC, AP_KH, AP_N
-};
+static_assert(AF_N % 2 == 0, "a point record is read as 16-byte pairs");
 
 struct AhLevel {
   KGrid G;
   double rmin, rmax, zmin, zmax;      // rminl .. zmaxl (evolve.f90:80-95)
-  const double* p[AF_N];              // plane of every interpolated quantity
+  const double* q;                    // AF_N values per point, point index IDX(G, i, j)
 };
 
 struct AhCtx {
@@ -106,7 +105,6 @@ __global__ void __launch_bounds__(256) k_ah_fields(KGrid G, const double* __rest
   const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
   if (j > G.Nz || i > G.Nr) return;
-  const long pl = G.plane;
   const GlobalView V{y, G, nullptr};
   const double r = (static_cast<double>(i) - 0.5) * G.dr;
   const double chi = V.fld(S_chi)(i, j);
@@ -131,21 +129,41 @@ __global__ void __launch_bounds__(256) k_ah_fields(KGrid G, const double* __rest
   const double KTH = -(gA * KTA + gB * KTB + 2.0 * r * r * gC * KTC) / gH;
   const double psi2 = psi * psi, psi4 = psi2 * psi2;
   const double third = 1.0 / 3.0;
-  put(out + AP_psi * pl, G, i, j, psi, 1.0, 1.0);
-  put(out + AP_Drphi * pl, G, i, j, Drp, -1.0, 1.0);
-  put(out + AP_Dzphi * pl, G, i, j, Dzp, 1.0, -1.0);
-  put(out + AP_DrA * pl, G, i, j, dA.r, -1.0, 1.0);
-  put(out + AP_DzA * pl, G, i, j, dA.z, 1.0, -1.0);
-  put(out + AP_DrB * pl, G, i, j, dB.r, -1.0, 1.0);
-  put(out + AP_DzB * pl, G, i, j, dB.z, 1.0, -1.0);
-  put(out + AP_DrC * pl, G, i, j, dC.r, -1.0, -1.0);
-  put(out + AP_DzC * pl, G, i, j, dC.z, 1.0, 1.0);
-  put(out + AP_DrH * pl, G, i, j, dH.r, -1.0, 1.0);
-  put(out + AP_DzH * pl, G, i, j, dH.z, 1.0, -1.0);
-  put(out + AP_KA * pl, G, i, j, psi4 * (KTA + third * A * trK), 1.0, 1.0);
-  put(out + AP_KB * pl, G, i, j, psi4 * (KTB + third * B * trK), 1.0, 1.0);
-  put(out + AP_KC * pl, G, i, j, psi4 * (KTC + third * C * trK), 1.0, -1.0);
-  put(out + AP_KH * pl, G, i, j, psi4 * (KTH + third * H * trK), 1.0, 1.0);
+  double v[AF_N];
+  v[AF_psi] = psi;
+  v[AF_Drphi] = Drp;
+  v[AF_Dzphi] = Dzp;
+  v[AF_A] = A;
+  v[AF_DrA] = dA.r;
+  v[AF_DzA] = dA.z;
+  v[AF_B] = B;
+  v[AF_DrB] = dB.r;
+  v[AF_DzB] = dB.z;
+  v[AF_C] = C;
+  v[AF_DrC] = dC.r;
+  v[AF_DzC] = dC.z;
+  v[AF_H] = H;
+  v[AF_DrH] = dH.r;
+  v[AF_DzH] = dH.z;
+  v[AF_trK] = trK;
+  v[AF_KA] = psi4 * (KTA + third * A * trK);
+  v[AF_KB] = psi4 * (KTB + third * B * trK);
+  v[AF_KC] = psi4 * (KTC + third * C * trK);
+  v[AF_KH] = psi4 * (KTH + third * H * trK);
+  // parities (arrays.config; a derivative flips its direction's parity)
+  constexpr double SR[AF_N] = {1, -1, 1, 1, -1, 1, 1, -1, 1, 1, -1, 1, 1, -1, 1, 1, 1, 1, 1, 1};
+  constexpr double SZ[AF_N] = {1, 1, -1, 1, 1, -1, 1, 1, -1, -1, -1, 1, 1, 1, -1, 1, 1, 1, -1, 1};
+  auto write = [&](int ii, int jj, bool fr, bool fz) {
+    double* o = out + IDX(G, ii, jj) * AF_N;
+#pragma unroll
+    for (int f = 0; f < AF_N; ++f) o[f] = v[f] * (fr ? SR[f] : 1.0) * (fz ? SZ[f] : 1.0);
+  };
+  write(i, j, false, false);
+  if (i <= G.g) write(1 - i, j, true, false);
+  if (G.eqsym && j >= 1 && j <= G.g) {
+    write(i, 1 - j, false, true);
+    if (i <= G.g) write(1 - i, 1 - j, true, true);
+  }
 }
 
 // ------------------------------------------------------------ interpolation
@@ -165,24 +183,30 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
   const double deltaz = z0 - (static_cast<double>(j0) - G.zoff) * G.dz;
   auto ci = [&](int i) { return i < 1 - g ? 1 - g : (i > G.Nr ? G.Nr : i); };
   auto cj = [&](int j) { return j < 1 - g ? 1 - g : (j > G.Nz ? G.Nz : j); };
+  // one point record: AF_N doubles read as 16-byte pairs
+  auto rec = [&](int i, int j) { return reinterpret_cast<const double2*>(L.q + IDX(G, ci(i), cj(j)) * AF_N); };
   if (deltar == 0.0 && deltaz == 0.0) {
-    const long k = IDX(G, ci(i0), cj(j0));
+    const double2* r = rec(i0, j0);
 #pragma unroll
-    for (int f = 0; f < AF_N; ++f) v[f] = L.p[f][k];
+    for (int f = 0; f < AF_N / 2; ++f) {
+      const double2 t = r[f];
+      v[2 * f] = t.x;
+      v[2 * f + 1] = t.y;
+    }
     return;
   }
   const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
                    ((z0 - L.zmin) < G.dz);
   const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
   if (order == OA_INTERP_BILINEAR || lin) {
-    const long k00 = IDX(G, ci(i0), cj(j0)), k10 = IDX(G, ci(i0 + 1), cj(j0));
-    const long k01 = IDX(G, ci(i0), cj(j0 + 1)), k11 = IDX(G, ci(i0 + 1), cj(j0 + 1));
+    const double* r00 = reinterpret_cast<const double*>(rec(i0, j0));
+    const double* r10 = reinterpret_cast<const double*>(rec(i0 + 1, j0));
+    const double* r01 = reinterpret_cast<const double*>(rec(i0, j0 + 1));
+    const double* r11 = reinterpret_cast<const double*>(rec(i0 + 1, j0 + 1));
 #pragma unroll
-    for (int f = 0; f < AF_N; ++f) {
-      const double* p = L.p[f];
-      v[f] = p[k00] * (1.0 - auxr) * (1.0 - auxz) + p[k10] * auxr * (1.0 - auxz) + p[k01] * (1.0 - auxr) * auxz +
-             p[k11] * auxr * auxz;
-    }
+    for (int f = 0; f < AF_N; ++f)
+      v[f] = r00[f] * (1.0 - auxr) * (1.0 - auxz) + r10[f] * auxr * (1.0 - auxz) + r01[f] * (1.0 - auxr) * auxz +
+             r11[f] * auxr * auxz;
     return;
   }
   double w[4][4];
@@ -205,24 +229,21 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
       w[i + 1][j + 1] = aux;
     }
   }
-  // IDX(G, i, j) = row(i) + col(j)
-  long row[4];
-  int col[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    row[q] = static_cast<long>(ci(i0 - 1 + q) + g - 1) * G.ld;
-    col[q] = cj(j0 - 1 + q) + g - 1;
-  }
+  for (int f = 0; f < AF_N; ++f) v[f] = 0.0;
 #pragma unroll
-  for (int f = 0; f < AF_N; ++f) {
-    const double* __restrict__ p = L.p[f];
-    double out = 0.0;
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const double2* r = rec(i0 - 1 + i, j0 - 1 + j);
+      const double wij = w[i][j];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out = out + w[i][j] * p[row[i] + col[j]];
-    v[f] = out;
-  }
+      for (int f = 0; f < AF_N / 2; ++f) {
+        const double2 t = r[f];
+        v[2 * f] = v[2 * f] + wij * t.x;
+        v[2 * f + 1] = v[2 * f + 1] + wij * t.y;
+      }
+    }
 }
 
 // ------------------------------------------------------------ ah_source
@@ -554,7 +575,7 @@ uint64_t key_of(double z) {
 // shot tables, ahfound(k) (kept between searches as the reference keeps it)
 // and the last integration of each horizon.
 struct AhState {
-  std::vector<double*> planes;            // AP_N planes per level
+  std::vector<double*> planes;            // point records of every level
   AhCtx hctx{};
   AhCtx* dctx = nullptr;
   ShotIn* h_in = nullptr;                 // pinned
@@ -562,8 +583,8 @@ struct AhState {
   ShotIn* d_in = nullptr;
   ShotOut* d_out = nullptr;
   int cap = 0;
-  double* d_theta = nullptr;              // one recorded integration
-  double* d_rad = nullptr;
+  double* d_theta[3] = {nullptr, nullptr, nullptr};   // last integration of horizon k
+  double* d_rad[3] = {nullptr, nullptr, nullptr};
   int found[3] = {0, 0, 0};
   std::vector<double> theta[3], rad[3];
 };
@@ -602,28 +623,21 @@ int ensure_capacity(AhState& a, int n) {
   return OA_OK;
 }
 
-// Integrate the shots zs (all centred at zcenter) in one launch.
-int run_shots(oa_handle* h, AhState& a, const std::vector<double>& zs, double zcenter, bool record,
-              std::vector<ShotRes>& res) {
-  const int n = static_cast<int>(zs.size());
+// Integrate the shots `in` in one launch, staged through buf's tables.
+int run_shots(hipStream_t st, AhState& buf, const std::vector<ShotIn>& in, std::vector<ShotRes>& res) {
+  const int n = static_cast<int>(in.size());
   res.assign(static_cast<size_t>(n), ShotRes{0.0, 0.0, 0, 0});
   if (n == 0) return OA_OK;
-  AH_RC(ensure_capacity(a, n));
-  for (int t = 0; t < n; ++t) {
-    a.h_in[t].ctx = a.dctx;
-    a.h_in[t].zinit = zs[static_cast<size_t>(t)];
-    a.h_in[t].zcenter = zcenter;
-    a.h_in[t].theta = record ? a.d_theta : nullptr;
-    a.h_in[t].rad = record ? a.d_rad : nullptr;
-  }
-  hipStream_t st = h->stream;
-  AH_HIP(hipMemcpyAsync(a.d_in, a.h_in, sizeof(ShotIn) * n, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_ah_shoot, dim3((n + kShotBlock - 1) / kShotBlock), dim3(kShotBlock), 0, st, a.d_in, a.d_out, n);
+  AH_RC(ensure_capacity(buf, n));
+  std::memcpy(buf.h_in, in.data(), sizeof(ShotIn) * n);
+  AH_HIP(hipMemcpyAsync(buf.d_in, buf.h_in, sizeof(ShotIn) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_ah_shoot, dim3((n + kShotBlock - 1) / kShotBlock), dim3(kShotBlock), 0, st, buf.d_in, buf.d_out,
+                     n);
   AH_HIP(hipGetLastError());
-  AH_HIP(hipMemcpyAsync(a.h_out, a.d_out, sizeof(ShotOut) * n, hipMemcpyDeviceToHost, st));
+  AH_HIP(hipMemcpyAsync(buf.h_out, buf.d_out, sizeof(ShotOut) * n, hipMemcpyDeviceToHost, st));
   AH_HIP(hipStreamSynchronize(st));
   for (int t = 0; t < n; ++t)
-    res[static_cast<size_t>(t)] = ShotRes{a.h_out[t].dradlast, a.h_out[t].area, a.h_out[t].err, a.h_out[t].ntheta};
+    res[static_cast<size_t>(t)] = ShotRes{buf.h_out[t].dradlast, buf.h_out[t].area, buf.h_out[t].err, buf.h_out[t].ntheta};
   return OA_OK;
 }
 
@@ -753,7 +767,7 @@ int build_fields(oa_handle* h, AhState& a) {
   if (static_cast<int>(a.planes.size()) != nlev) {
     a.planes.assign(static_cast<size_t>(nlev), nullptr);
     for (int l = 0; l < nlev; ++l) {
-      const size_t bytes = sizeof(double) * AP_N * static_cast<size_t>(h->L[l].G.plane);
+      const size_t bytes = sizeof(double) * AF_N * static_cast<size_t>(h->L[l].G.plane);
       if (hipMalloc(reinterpret_cast<void**>(&a.planes[static_cast<size_t>(l)]), bytes) != hipSuccess) {
         oa_set_error("oa_ahfind: device allocation of the horizon planes failed");
         return OA_ERR_NOMEM;
@@ -761,8 +775,6 @@ int build_fields(oa_handle* h, AhState& a) {
       AH_HIP(hipMemsetAsync(a.planes[static_cast<size_t>(l)], 0, bytes, h->stream));
     }
     AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.dctx), sizeof(AhCtx)));
-    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_theta), sizeof(double) * (kAhNthetaMax + 1)));
-    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_rad), sizeof(double) * (kAhNthetaMax + 1)));
   }
   const oa_params& p = h->p;
   AhCtx& c = a.hctx;
@@ -790,13 +802,7 @@ int build_fields(oa_handle* h, AhState& a) {
     A.rmax = (static_cast<double>(G.Nr) - 0.5) * G.dr;
     A.zmin = (static_cast<double>(1 - g) - G.zoff) * G.dz;
     A.zmax = (static_cast<double>(G.Nz) - G.zoff) * G.dz;
-    const long pl = G.plane;
-    const int derived[AF_N] = {AP_psi, AP_Drphi, AP_Dzphi, -S_A, AP_DrA, AP_DzA, -S_B, AP_DrB, AP_DzB,
-                               -S_C, AP_DrC, AP_DzC, -S_H, AP_DrH, AP_DzH, -S_trK, AP_KA, AP_KB, AP_KC, AP_KH};
-    for (int f = 0; f < AF_N; ++f) {
-      const bool state = (f == AF_A || f == AF_B || f == AF_C || f == AF_H || f == AF_trK);
-      A.p[f] = state ? y + static_cast<long>(-derived[f]) * pl : out + static_cast<long>(derived[f]) * pl;
-    }
+    A.q = out;
   }
   AH_HIP(hipMemcpyAsync(a.dctx, &a.hctx, sizeof(AhCtx), hipMemcpyHostToDevice, h->stream));
   return OA_OK;
@@ -804,14 +810,25 @@ int build_fields(oa_handle* h, AhState& a) {
 
 }  // namespace
 
+int ah_record_doubles() { return AF_N; }
+
+// the context, the recorded last integration of each horizon and the shot
+// tables at their usual size (one march chunk)
+double ah_fixed_bytes(int n_ah) {
+  return static_cast<double>(sizeof(AhCtx)) + 2.0 * n_ah * sizeof(double) * (kAhNthetaMax + 1) +
+         static_cast<double>(kAhScanChunk) * 2.0 * (sizeof(ShotIn) + sizeof(ShotOut));
+}
+
 void ah_release(oa_handle* h) {
   AhState* a = h->ah;
   if (!a) return;
   for (double* p : a->planes)
     if (p) (void)hipFree(p);
   if (a->dctx) (void)hipFree(a->dctx);
-  if (a->d_theta) (void)hipFree(a->d_theta);
-  if (a->d_rad) (void)hipFree(a->d_rad);
+  for (int k = 0; k < 3; ++k) {
+    if (a->d_theta[k]) (void)hipFree(a->d_theta[k]);
+    if (a->d_rad[k]) (void)hipFree(a->d_rad[k]);
+  }
   if (a->d_in) (void)hipFree(a->d_in);
   if (a->d_out) (void)hipFree(a->d_out);
   if (a->h_in) (void)hipHostFree(a->h_in);
@@ -824,21 +841,20 @@ void ah_release(oa_handle* h) {
 
 using namespace oax;
 
-extern "C" int oa_ahfind(oa_handle* h, int k, oa_horizon* out) {
-  if (!h || !out) {
-    oa_set_error("oa_ahfind: null argument");
-    return OA_ERR_ARG;
-  }
+namespace {
+
+// horizon_finder's checks (:48-95) and this library's scope
+int ah_check(const oa_handle* h, int k) {
   const oa_params& p = h->p;
-  if (p.N_ah > 3 || p.N_ah < 1 || k < 0 || k >= p.N_ah) {                  // :48-57
+  if (p.N_ah > 3 || p.N_ah < 1 || k < 0 || k >= p.N_ah) {
     oa_set_error("oa_ahfind: horizon index out of range (N_ah <= 3)");
     return OA_ERR_ARG;
   }
-  if (p.eqsym && p.N_ah == 3) {                                              // :62-71
+  if (p.eqsym && p.N_ah == 3) {
     oa_set_error("oa_ahfind: for equatorial symmetry we can only look for 2 horizons");
     return OA_ERR_PARAM;
   }
-  if (p.z_ah1 < 0.0 || p.z_ah2 > 0.0) {                                       // :76-95
+  if (p.z_ah1 < 0.0 || p.z_ah2 > 0.0) {
     oa_set_error("oa_ahfind: horizon 1 must be above the equator and horizon 2 below");
     return OA_ERR_PARAM;
   }
@@ -855,73 +871,166 @@ extern "C" int oa_ahfind(oa_handle* h, int k, oa_horizon* out) {
       oa_set_error("oa_ahfind: state not projected (call oa_auxiliary / oa_initial_aux)");
       return OA_ERR_STATE;
     }
-  if (!h->ah) h->ah = new AhState();
-  AhState& a = *h->ah;
-  AH_RC(build_fields(h, a));
+  return OA_OK;
+}
 
-  double zah[3] = {0.0, 0.0, 0.0};                                           // :104-111
-  if (p.N_ah > 1) {
-    zah[1] = p.z_ah1;
-    zah[2] = p.z_ah2;
-  }
-  const double zcenter = zah[k];
-  const AhLevel& L0 = a.hctx.L[0];
+struct Job {
+  oa_handle* h;
+  int k;
   Search s;
-  s.c = &a.hctx;
-  s.zcenter = zcenter;
-  s.eps = p.ahepsilon_shoot;
-  s.found_in = a.found[k];
-  if (p.ahzinit == 0.0) {                                                    // :379-405
-    s.zinit = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
-  } else if (p.ahzinit > 0.0 && p.ahzinit <= 0.9 * L0.zmax) {
-    s.zinit = (zcenter >= 0.0) ? p.ahzinit : -p.ahzinit;
-  } else {
-    s.zinit = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
+  double zstart;
+  bool done;
+};
+
+// The searches of `jobs` in lockstep on stream st: every round integrates
+// the speculative shots of all unfinished searches in one launch.  Then the
+// last integration of each search is recorded (one more launch).
+int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
+  for (Job& j : jobs) {
+    AH_RC(ah_check(j.h, j.k));
+    if (!j.h->ah) j.h->ah = new AhState();
   }
-  const double zstart = s.zinit;
+  for (Job& j : jobs) {
+    AhState& a = *j.h->ah;
+    AH_RC(build_fields(j.h, a));
+    const oa_params& p = j.h->p;
+    double zah[3] = {0.0, 0.0, 0.0};                                         // :104-111
+    if (p.N_ah > 1) {
+      zah[1] = p.z_ah1;
+      zah[2] = p.z_ah2;
+    }
+    const double zcenter = zah[j.k];
+    const AhLevel& L0 = a.hctx.L[0];
+    j.s.c = &a.hctx;
+    j.s.zcenter = zcenter;
+    j.s.eps = p.ahepsilon_shoot;
+    j.s.found_in = a.found[j.k];
+    if (p.ahzinit == 0.0) {                                                  // :379-405
+      j.zstart = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
+    } else if (p.ahzinit > 0.0 && p.ahzinit <= 0.9 * L0.zmax) {
+      j.zstart = (zcenter >= 0.0) ? p.ahzinit : -p.ahzinit;
+    } else {
+      j.zstart = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
+    }
+    j.done = false;
+  }
+  AhState& buf = *jobs[0].h->ah;
   std::vector<double> want;
+  std::vector<ShotIn> in;
+  std::vector<std::pair<size_t, double>> owner;   // (job, zinit) of every launched shot
   std::vector<ShotRes> res;
   for (;;) {
-    s.zinit = zstart;
-    if (walk(s, want)) break;
-    AH_RC(run_shots(h, a, want, zcenter, false, res));
-    for (size_t q = 0; q < want.size(); ++q) s.table[key_of(want[q])] = res[q];
+    in.clear();
+    owner.clear();
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      Job& j = jobs[q];
+      if (j.done) continue;
+      j.s.zinit = j.zstart;
+      if (walk(j.s, want)) {
+        j.done = true;
+        continue;
+      }
+      for (double z : want) {
+        in.push_back(ShotIn{j.h->ah->dctx, z, j.s.zcenter, nullptr, nullptr});
+        owner.push_back({q, z});
+      }
+    }
+    if (in.empty()) break;
+    AH_RC(run_shots(st, buf, in, res));
+    for (size_t t = 0; t < in.size(); ++t) jobs[owner[t].first].s.table[key_of(owner[t].second)] = res[t];
   }
-  a.found[k] = s.found;
-  // the last integration, recorded (ah_radius_SH, :956-966)
-  if (s.shots > 0) {
-    AH_RC(run_shots(h, a, std::vector<double>{s.last_zinit}, zcenter, true, res));
-    const int n = res[0].ntheta + 1;
-    a.theta[k].resize(static_cast<size_t>(n));
-    a.rad[k].resize(static_cast<size_t>(n));
-    AH_HIP(hipMemcpyAsync(a.theta[k].data(), a.d_theta, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
-    AH_HIP(hipMemcpyAsync(a.rad[k].data(), a.d_rad, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
-    AH_HIP(hipStreamSynchronize(h->stream));
-  } else {
-    a.theta[k].clear();
-    a.rad[k].clear();
+  // the last integration of every search, recorded (ah_radius_SH, :956-966)
+  in.clear();
+  owner.clear();
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    Job& j = jobs[q];
+    AhState& a = *j.h->ah;
+    a.theta[j.k].clear();
+    a.rad[j.k].clear();
+    if (j.s.shots == 0) continue;
+    if (!a.d_theta[j.k]) {
+      AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_theta[j.k]), sizeof(double) * (kAhNthetaMax + 1)));
+      AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_rad[j.k]), sizeof(double) * (kAhNthetaMax + 1)));
+    }
+    in.push_back(ShotIn{a.dctx, j.s.last_zinit, j.s.zcenter, a.d_theta[j.k], a.d_rad[j.k]});
+    owner.push_back({q, j.s.last_zinit});
   }
-  if (!s.found)
-    for (double& v : a.rad[k]) v = 0.0;
+  AH_RC(run_shots(st, buf, in, res));
+  for (size_t t = 0; t < in.size(); ++t) {
+    Job& j = jobs[owner[t].first];
+    AhState& a = *j.h->ah;
+    const int n = res[t].ntheta + 1;
+    a.theta[j.k].resize(static_cast<size_t>(n));
+    a.rad[j.k].resize(static_cast<size_t>(n));
+    AH_HIP(hipMemcpyAsync(a.theta[j.k].data(), a.d_theta[j.k], sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    AH_HIP(hipMemcpyAsync(a.rad[j.k].data(), a.d_rad[j.k], sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  }
+  AH_HIP(hipStreamSynchronize(st));
 
   const double smallpi = M_PI, half = 0.5, two = 2.0;
-  std::memset(out, 0, sizeof *out);
-  out->found = s.found;
-  out->ntheta = s.last.ntheta;
-  out->shots = s.shots;
-  out->errorflag = s.errorflag;
-  out->zcenter = zcenter;
-  out->zinit = s.zinit;
-  out->dradlast = s.dradlast;
-  double ah_area = 0.0;                                                      // :989-997
-  if (s.found) {
-    ah_area = s.last.area;
-    if (p.eqsym && zcenter == 0.0) ah_area = two * ah_area;
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    Job& j = jobs[q];
+    const Search& s = j.s;
+    AhState& a = *j.h->ah;
+    a.found[j.k] = s.found;
+    if (!s.found)
+      for (double& v : a.rad[j.k]) v = 0.0;
+    oa_horizon& o = out[q];
+    std::memset(&o, 0, sizeof o);
+    o.found = s.found;
+    o.ntheta = s.last.ntheta;
+    o.shots = s.shots;
+    o.errorflag = s.errorflag;
+    o.zcenter = s.zcenter;
+    o.zinit = s.zinit;
+    o.dradlast = s.dradlast;
+    double ah_area = 0.0;                                                    // :989-997
+    if (s.found) {
+      ah_area = s.last.area;
+      if (j.h->p.eqsym && s.zcenter == 0.0) ah_area = two * ah_area;
+    }
+    o.area = ah_area;
+    o.radius = half * std::sqrt(ah_area / smallpi);
+    o.mass = s.found ? half * o.radius : 0.0;                                // :1065-1076
   }
-  out->area = ah_area;
-  out->radius = half * std::sqrt(ah_area / smallpi);
-  out->mass = s.found ? half * out->radius : 0.0;                            // :1065-1076
   return OA_OK;
+}
+
+}  // namespace
+
+extern "C" int oa_ahfind(oa_handle* h, int k, oa_horizon* out) {
+  if (!h || !out) {
+    oa_set_error("oa_ahfind: null argument");
+    return OA_ERR_ARG;
+  }
+  std::vector<Job> jobs(1);
+  jobs[0].h = h;
+  jobs[0].k = k;
+  return search_many(jobs, h->stream, out);
+}
+
+extern "C" int oa_batch_ahfind(oa_batch* b, int n, oa_handle* const* members, const int* k, oa_horizon* out) {
+  if (!b || n < 0 || (n > 0 && (!members || !k || !out))) {
+    oa_set_error("oa_batch_ahfind: bad argument");
+    return OA_ERR_ARG;
+  }
+  if (n == 0) return OA_OK;
+  std::vector<Job> jobs(static_cast<size_t>(n));
+  for (int q = 0; q < n; ++q) {
+    oa_handle* h = members[q];
+    if (!h || h->batch != b) {
+      oa_set_error("oa_batch_ahfind: every spacetime must be a member of the batch");
+      return OA_ERR_ARG;
+    }
+    for (int r = 0; r < q; ++r)
+      if (members[r] == h && k[r] == k[q]) {
+        oa_set_error("oa_batch_ahfind: the same horizon of a spacetime twice");
+        return OA_ERR_ARG;
+      }
+    jobs[static_cast<size_t>(q)].h = h;
+    jobs[static_cast<size_t>(q)].k = k[q];
+  }
+  return search_many(jobs, members[0]->stream, out);
 }
 
 extern "C" int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n) {

@@ -27,7 +27,7 @@ module oaxis_mod
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
             oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule, &
-            oa_constraints, oa_ahfind, oa_horizon_shape
+            oa_constraints, oa_ahfind, oa_horizon_shape, oa_batch_ahfind
 
   ! result of one apparent-horizon search (oa_horizon in include/oaxis.h)
   type, bind(C), public :: oa_horizon
@@ -257,6 +257,15 @@ module oaxis_mod
       integer(c_int), value :: k, cap
       real(c_double), dimension(*) :: theta, rad
       integer(c_int) :: n
+    end function
+    ! horizon_finder for horizon k(i) of members(i), i <= n, in lockstep (no reference counterpart)
+    integer(c_int) function oa_batch_ahfind(b, n, members, k, res) bind(C, name="oa_batch_ahfind")
+      import :: c_int, c_ptr, oa_horizon
+      type(c_ptr), value :: b
+      integer(c_int), value :: n
+      type(c_ptr), dimension(*) :: members
+      integer(c_int), dimension(*) :: k
+      type(oa_horizon), dimension(*) :: res
     end function
     ! 0D norms (max, min, nm1, nm2) of an evolved array, as save0Ddata forms them
     integer(c_int) function oa_norms0d(h, box, level, name, norms) bind(C, name="oa_norms0d")

@@ -282,6 +282,19 @@ class Batch:
     def sync(self):
         _check(lib().oa_batch_sync(self._b), "oa_batch_sync")
 
+    def ahfind(self, searches):
+        """horizon_finder for every (spacetime, k) in ``searches`` (members of
+        this batch) in lockstep; a list of dicts as Spacetime.ahfind returns."""
+        n = len(searches)
+        if n == 0:
+            return []
+        hs = (ctypes.c_void_p * n)(*[st._h.value if isinstance(st._h, ctypes.c_void_p) else st._h
+                                     for st, _ in searches])
+        ks = (ctypes.c_int * n)(*[int(k) for _, k in searches])
+        res = (_abi.Horizon * n)()
+        _check(lib().oa_batch_ahfind(self._b, n, hs, ks, res), "oa_batch_ahfind")
+        return [{name: getattr(r, name) for name, _ in _abi.Horizon._fields_} for r in res]
+
     def last_elapsed_ms(self):
         v = ctypes.c_double()
         _check(lib().oa_batch_last_elapsed_ms(self._b, ctypes.byref(v)), "oa_batch_last_elapsed_ms")

@@ -5,13 +5,20 @@ as they arrive while their device footprint fits the memory budget and a
 member slot is free.  Every admitted spacetime keeps its state on the device
 and is advanced inside one Batch, so requests that arrive while others run
 join the same launches.  A request retires when it has done its coarse
-steps (the reference's Nt), when its lapse collapses (the build's stand-in
-for horizon formation, checked every ``check_every`` coarse steps: min alpha
-on the finest level falls below ``alpha_collapse``, for requests whose lapse
-starts above it) or when its scalar field has
-dispersed (max |scalar_phi| on level 0 below ``disperse_frac`` of its
-initial value).  The reference has no serving loop; the retirement criteria
-other than the step count are defined here.
+steps (the reference's Nt), when an apparent horizon forms, or when its
+scalar field has dispersed (max |scalar_phi| on level 0 below
+``disperse_frac`` of its initial value).
+
+Horizons: a request with ``ahfind = .true.`` is searched as the reference
+does (horizon_finder at t = 0 and every ``ahfind_every`` coarse steps once
+t >= ``ahafter``, evolve.f90:251 and :429-432), all due searches of one step
+in one lockstep batch (oa_batch_ahfind).  It retires ("horizon") when a
+horizon is found that was not there at t = 0 (a puncture's horizon exists
+from the start and does not end the request).  A request without ahfind
+retires when its lapse collapses instead (checked every ``check_every``
+coarse steps: min alpha on the finest level below ``alpha_collapse``, for
+requests whose lapse starts above it).  The reference has no serving loop;
+the retirement criteria other than the step count are defined here.
 
 Completion latency is retirement time minus arrival time; run() returns one
 record per request and the aggregate throughput.
@@ -43,6 +50,7 @@ class Record:
     point_updates: float = 0.0
     footprint: float = 0.0
     norms: dict = field(default_factory=dict)
+    horizon: dict = field(default_factory=dict)   # the oa_ahfind result that retired it
 
     @property
     def latency(self):
@@ -71,12 +79,19 @@ class Server:
         rec = Record(req.rid, req.kind, req.arrival, admitted=now, footprint=fp)
         scalar = bool(p.mattertype & oaxis.ENUMS["OA_MATTER_SCALAR"])
         phi0 = st.norms0d("scalar_phi", 0)["nm1"] if scalar else 0.0
-        # only a collapse that forms during the evolution retires a request
-        # (a puncture's lapse starts near zero)
-        watch = st.norms0d("alpha", st.nlevels - 1)["min"] >= self.alpha_collapse
+        ah = bool(p.ahfind)
+        if ah:
+            # the search at t = 0 (evolve.f90:251): only a horizon that forms
+            # later retires the request
+            watch = not (p.ahafter <= 0.0 and st.ahfind(0)["found"])
+        else:
+            # only a collapse that forms during the evolution retires a
+            # request (a puncture's lapse starts near zero)
+            watch = st.norms0d("alpha", st.nlevels - 1)["min"] >= self.alpha_collapse
         batch.add(st)
-        active.append(dict(req=req, st=st, rec=rec, steps=0, scalar=scalar, phi0=phi0, watch=watch,
-                           u0=st.point_updates()))
+        active.append(dict(req=req, st=st, rec=rec, steps=0, scalar=scalar, phi0=phi0, watch=watch, ah=ah,
+                           ah_every=max(1, int(p.ahfind_every)), ah_after=float(p.ahafter),
+                           dt0=st.level_info(0)["dt"], u0=st.point_updates()))
         return used + fp
 
     def _retire(self, a, reason, now, batch):
@@ -100,7 +115,7 @@ class Server:
         st = a["st"]
         lev = st.nlevels - 1
         try:
-            if a["watch"] and st.norms0d("alpha", lev)["min"] < self.alpha_collapse:
+            if a["watch"] and not a["ah"] and st.norms0d("alpha", lev)["min"] < self.alpha_collapse:
                 return "lapse_collapse"
             if a["scalar"] and a["phi0"] > 0.0:
                 if st.norms0d("scalar_phi", 0)["nm1"] < self.disperse_frac * a["phi0"]:
@@ -141,13 +156,30 @@ class Server:
                 for a in active:
                     a["steps"] += 1
                 retire = []
+                due = []
                 for a in active:
                     if a["steps"] >= a["req"].steps:
                         retire.append((a, "final_time"))
-                    elif self.check_every > 0 and a["steps"] % self.check_every == 0:
+                        continue
+                    if self.check_every > 0 and a["steps"] % self.check_every == 0:
                         reason = self._check(a)
                         if reason:
                             retire.append((a, reason))
+                            continue
+                    if (a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0
+                            and a["steps"] * a["dt0"] >= a["ah_after"]):
+                        due.append(a)
+                if due:
+                    try:
+                        found = batch.ahfind([(a["st"], 0) for a in due])
+                    except oaxis.OaxisError:
+                        found = None
+                    for q, a in enumerate(due):
+                        if found is None:
+                            retire.append((a, "failed"))
+                        elif found[q]["found"]:
+                            a["rec"].horizon = found[q]
+                            retire.append((a, "horizon"))
                 if retire:
                     batch.sync()
                     now = clock() - t0
@@ -171,11 +203,12 @@ class Server:
 
 
 def serving_mix(seed, n, rate, grids=((64, 128), (128, 256), (256, 512), (512, 1024)), max_levels=6,
-                steps=(200, 2000)):
+                steps=(200, 2000), ahfind_every=16):
     """Seeded request stream of the serving mix: Poisson arrivals at ``rate``
     per second, 40 % Schwarzschild (M~U[.5,2]), 30 % Brill (a~U[1,4]), 30 %
     scalar pulse; grid, level count (1..max_levels) and coarse steps drawn
-    uniformly."""
+    uniformly.  Every request searches for horizons every ``ahfind_every``
+    coarse steps (0: no horizon search)."""
     from . import workloads
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -196,5 +229,7 @@ def serving_mix(seed, n, rate, grids=((64, 128), (128, 256), (256, 512), (512, 1
             kind = "scalar_pulse"
             text = workloads.scalar_pulse(seed, k) + "\nNrtotal = %d\nNztotal = %d\nNl0 = %d\ndr0 = %r\ndz0 = %r\n" % (
                 Nr, Nz, nl0, 12.0 / Nr, 12.0 / Nr)
+        if ahfind_every > 0:
+            text += "\nahfind = .true.\nahfind_every = %d\n" % ahfind_every
         out.append(Request(k, text, nsteps, t, kind))
     return out

@@ -144,3 +144,77 @@ def test_evolved_two_levels_match_oracle(oaxis_mod, oracle_mod, steps):
         assert again[key] == dev[key]
     st.close()
     orc.close()
+
+
+def _schwarzschild_text(M, extra=""):
+    return EVOLVED.replace("BH1mass = 2.0", "BH1mass = %r" % M) + extra
+
+
+def test_batch_search_equals_alone(oaxis_mod):
+    """oa_batch_ahfind runs the searches of several members in lockstep; each
+    result equals that member's search alone (separate handles, same state)."""
+    oaxis = oaxis_mod
+    texts = [_schwarzschild_text(2.0), _schwarzschild_text(1.6, "Nl0 = 0\n"),
+             _schwarzschild_text(2.0, "N_ah = 2\nz_ah1 = 0.0\n"), CASES_FLAT]
+    ks = [0, 0, 1, 0]
+    alone, members = [], []
+    for text, k in zip(texts, ks):
+        p = oaxis.params_from_text(text)
+        with oaxis.Spacetime(p) as st:
+            st.initial_data()
+            st.initial_aux()
+            st.evolve(2)
+            alone.append(st.ahfind(k, shape=True))
+        st = oaxis.Spacetime(p)
+        st.initial_data()
+        st.initial_aux()
+        members.append(st)
+    batch = oaxis.Batch(members)
+    batch.evolve(2)
+    t0 = time.perf_counter()
+    got = batch.ahfind(list(zip(members, ks)))
+    t1 = time.perf_counter()
+    print("batch of %d searches: %.3f s" % (len(got), t1 - t0))
+    for q, (g, a) in enumerate(zip(got, alone)):
+        for key in EXACT + ("area", "mass", "dradlast"):
+            assert g[key] == a[key], (q, key, g[key], a[key])
+    assert [g["found"] for g in got] == [1, 1, 1, 0]
+    batch.close()
+    for st in members:
+        st.close()
+
+
+CASES_FLAT = """
+order = four
+dtfac = 0.25
+boundtype = radiative
+geodiss = 0.1
+idata = minkowski
+slicing = 1+log
+gauge_f = 2.0
+integrator = rk4
+eqsym = .false.
+dr0 = 0.1
+dz0 = 0.1
+Nrtotal = 32
+Nztotal = 64
+"""
+
+
+def test_serving_retires_on_horizon(oaxis_mod):
+    """With ahafter > 0 the search at t = 0 is skipped, so the puncture's
+    horizon counts as formed at the first search after ahafter: the request
+    retires there with reason "horizon"; with ahafter = 0 the horizon exists
+    from t = 0 and the request runs to its last step."""
+    oaxis = oaxis_mod
+    from oaxis.serve import Request, Server
+    late = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\nahafter = 0.1\n")
+    early = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\n")
+    reqs = [Request(0, late, 12, 0.0, "late"), Request(1, early, 6, 0.0, "early")]
+    dt0 = oaxis.params_from_text(late).dtfac * 0.1
+    records, summary = Server(1e12, check_every=0).run(reqs)
+    by = {r.rid: r for r in records}
+    first = next(s for s in range(2, 13, 2) if s * dt0 >= 0.1)
+    assert by[0].reason == "horizon" and by[0].steps == first, (by[0].reason, by[0].steps, first)
+    assert by[0].horizon["found"] == 1 and abs(by[0].horizon["mass"] - 2.0) < 1e-3
+    assert by[1].reason == "final_time" and by[1].steps == 6
