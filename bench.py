@@ -114,7 +114,8 @@ def copy_probe(nbytes):
 def run_serving(args, n, steps_scale, max_grid):
     """The serving mix through the continuous-batching loop: Poisson
     arrivals, admission by device footprint, retirement at the final step,
-    on lapse collapse or on dispersal.  Returns the loop's summary."""
+    on horizon formation (searched every 16 coarse steps) or on dispersal.
+    Returns the loop's summary."""
     from oaxis import serve
     grids = tuple(g for g in ((64, 128), (128, 256), (256, 512), (512, 1024)) if g[0] <= max_grid)
     lo, hi = max(1, int(200 * steps_scale)), max(1, int(2000 * steps_scale))
@@ -122,9 +123,41 @@ def run_serving(args, n, steps_scale, max_grid):
     srv = serve.Server(memory_budget=args.serving_budget_gb * 1e9)
     records, summary = srv.run(reqs)
     summary["sample"] = ("%d requests of the serving mix (seed %d, Poisson rate %.3g/s, grids up to %dx%d, "
-                         "1-6 levels, %d-%d coarse steps), memory budget %.0f GB"
+                         "1-6 levels, %d-%d coarse steps, horizon search every 16), memory budget %.0f GB"
                          % (n, args.seed, args.serving_rate, max_grid, 2 * max_grid, lo, hi, args.serving_budget_gb))
     return summary
+
+
+def horizon_probe(with_oracle):
+    """One apparent-horizon search (shooting method) on Schwarzschild M=2,
+    2 levels of 128x256: device time of oa_ahfind (second of two calls) and,
+    beside it, the oracle's search on the same state (one core)."""
+    import oaxis
+    from oaxis import workloads
+    text = workloads.schwarzschild(2.0, 128, 256, 1)
+    p = oaxis.params_from_text(text)
+    out = {"workload": "Schwarzschild M=2, 2 levels of 128x256, shooting method"}
+    with oaxis.Spacetime(p) as st:
+        st.initial_data()
+        st.initial_aux()
+        st.ahfind(0)
+        t0 = time.perf_counter()
+        r = st.ahfind(0)
+        out.update(device_ms=(time.perf_counter() - t0) * 1e3, found=r["found"], shots=r["shots"],
+                   mass=r["mass"])
+        if with_oracle:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+            orc = pyoracle.Oracle(p)
+            orc.initial_data()
+            orc.initial_aux()
+            t0 = time.perf_counter()
+            ro = orc.ahfind(0)
+            out["oracle_ms"] = (time.perf_counter() - t0) * 1e3
+            out["oracle_matches"] = bool(ro["shots"] == r["shots"] and ro["zinit"] == r["zinit"]
+                                         and ro["found"] == r["found"])
+            orc.close()
+    return out
 
 
 def run_batched(args):
@@ -216,6 +249,7 @@ def run_batched(args):
                      "launches": rhs_n,
                      "algorithmic_bytes_per_launch": rhs_bytes / rhs_n if rhs_n else 0.0},
     }
+    rec["horizon"] = horizon_probe(not args.no_cpu_baseline)
     if args.serving_requests > 0:
         sv = run_serving(args, args.serving_requests, args.serving_steps_scale, 256)
         rec["latency"] = {"p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "requests": sv["requests"],

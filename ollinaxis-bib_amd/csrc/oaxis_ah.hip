@@ -10,7 +10,7 @@
 // local cell until the end slope dh/dtheta changes sign, then bisecting.
 // Every shot is a serial adaptive RK4 over ~10^2-10^3 steps whose sources
 // interpolate 20 metric quantities, so one shot is a long dependent chain:
-// this file runs one shot per lane and many shots per launch.
+// this file runs one shot per wave and many shots per launch.
 //
 //  1. k_ah_fields: one pass per level forms the 20 quantities the ODE
 //     interpolates (psi, the conformal-factor and metric first derivatives,
@@ -19,13 +19,15 @@
 //     point by point (20 consecutive doubles per grid point): one bicubic
 //     stencil then reads 16 runs of 160 bytes instead of 320 scattered
 //     values.
-//  2. k_ah_shoot: one lane integrates one shot (zinit) to the end.
+//  2. k_ah_shoot: one wave integrates one shot (zinit) to the end; its lanes
+//     share the 320 loads of every interpolation, the rest is uniform.
 //  3. The host walks the reference's control flow (march, bracket, bisection,
 //     restart; ahshoot :434-855) over a table of shot results.  When the walk
 //     needs a shot it has not got, it launches the whole speculative
 //     continuation at once: during the march, the next positions down the
-//     axis; during the bisection, the full tree of the next kAhDepth
-//     bisection steps.  The walk then takes exactly the reference's path
+//     axis; during the bisection, the full tree of the next bisection steps,
+//     as deep as keeps every search's tree resident on the device at once
+//     (kAhResident waves in all, at most kAhMaxDepth steps).  The walk then takes exactly the reference's path
 //     through those results, so the sequence of shots and every decision
 //     are the reference's; unused shots are discarded.
 //
@@ -58,7 +60,8 @@ constexpr int kAhNthetaMax = 50000;                 // Nthetamax (:350)
 // bound is reached only by a shot the reference would integrate for hours.
 constexpr int kAhMaxSteps = 4 * kAhNthetaMax;
 constexpr int kAhScanChunk = 1024;                  // march positions per launch
-constexpr int kAhDepth = 11;                        // bisection steps per launch
+constexpr int kAhMaxDepth = 11;                     // bisection steps per launch, at most
+constexpr int kAhResident = 2048;                   // shot waves resident at once (256 CUs x 8)
 constexpr int kShotBlock = 64;
 
 // interpolated quantities, in ah_source's order
@@ -167,12 +170,16 @@ __global__ void __launch_bounds__(256) k_ah_fields(KGrid G, const double* __rest
 }
 
 // ------------------------------------------------------------ interpolation
-// interp (interp.f90:2-270) of every AF_* quantity at one point: one set of
-// Lagrange weights, formed in the reference's order, for all of them.  The
-// caller (ah_source) has already placed (r0, z0) at least one cell inside the
-// level (one cell inside level 0's edges); the stencil rows and columns are
-// clamped to the array, which matters only where a weight is exactly zero.
-__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N]) {
+// interp (interp.f90:2-270) of every AF_* quantity at one point, by the
+// whole wave of the shot: every lane forms the same Lagrange weights (in the
+// reference's order), lane f < AF_N sums quantity f over the stencil in the
+// reference's order -- so the 20 lanes read each point record as one
+// contiguous 160-byte access -- and the results reach every lane through
+// LDS (sv).  The caller (ah_source) has already placed (r0, z0) at least one
+// cell inside the level (one cell inside level 0's edges); the stencil rows
+// and columns are clamped to the array, which matters only where a weight
+// is exactly zero.
+__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N], double* sv) {
   const KGrid& G = L.G;
   const int g = G.g;
   const double r1g = (static_cast<double>(1 - g) - 0.5) * G.dr;
@@ -183,74 +190,66 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
   const double deltaz = z0 - (static_cast<double>(j0) - G.zoff) * G.dz;
   auto ci = [&](int i) { return i < 1 - g ? 1 - g : (i > G.Nr ? G.Nr : i); };
   auto cj = [&](int j) { return j < 1 - g ? 1 - g : (j > G.Nz ? G.Nz : j); };
-  // one point record: AF_N doubles read as 16-byte pairs
-  auto rec = [&](int i, int j) { return reinterpret_cast<const double2*>(L.q + IDX(G, ci(i), cj(j)) * AF_N); };
+  const int f = static_cast<int>(threadIdx.x);
+  // quantity f of one point record
+  auto at = [&](int i, int j) { return L.q[IDX(G, ci(i), cj(j)) * AF_N + f]; };
+  double out = 0.0;
   if (deltar == 0.0 && deltaz == 0.0) {
-    const double2* r = rec(i0, j0);
+    if (f < AF_N) out = at(i0, j0);
+  } else {
+    const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
+                     ((z0 - L.zmin) < G.dz);
+    const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
+    if (order == OA_INTERP_BILINEAR || lin) {
+      if (f < AF_N)
+        out = at(i0, j0) * (1.0 - auxr) * (1.0 - auxz) + at(i0 + 1, j0) * auxr * (1.0 - auxz) +
+              at(i0, j0 + 1) * (1.0 - auxr) * auxz + at(i0 + 1, j0 + 1) * auxr * auxz;
+    } else {
+      double w[4][4];
 #pragma unroll
-    for (int f = 0; f < AF_N / 2; ++f) {
-      const double2 t = r[f];
-      v[2 * f] = t.x;
-      v[2 * f + 1] = t.y;
-    }
-    return;
-  }
-  const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
-                   ((z0 - L.zmin) < G.dz);
-  const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
-  if (order == OA_INTERP_BILINEAR || lin) {
-    const double* r00 = reinterpret_cast<const double*>(rec(i0, j0));
-    const double* r10 = reinterpret_cast<const double*>(rec(i0 + 1, j0));
-    const double* r01 = reinterpret_cast<const double*>(rec(i0, j0 + 1));
-    const double* r11 = reinterpret_cast<const double*>(rec(i0 + 1, j0 + 1));
+      for (int i = -1; i <= 2; ++i) {
+        double wr = 1.0;
 #pragma unroll
-    for (int f = 0; f < AF_N; ++f)
-      v[f] = r00[f] * (1.0 - auxr) * (1.0 - auxz) + r10[f] * auxr * (1.0 - auxz) + r01[f] * (1.0 - auxr) * auxz +
-             r11[f] * auxr * auxz;
-    return;
-  }
-  double w[4][4];
+        for (int m = -1; m <= 2; ++m) {
+          if (m == i) continue;
+          wr = wr * (auxr - static_cast<double>(m)) / static_cast<double>(i - m);
+        }
 #pragma unroll
-  for (int i = -1; i <= 2; ++i) {
-    double wr = 1.0;
+        for (int j = -1; j <= 2; ++j) {
+          double aux = wr;
 #pragma unroll
-    for (int m = -1; m <= 2; ++m) {
-      if (m == i) continue;
-      wr = wr * (auxr - static_cast<double>(m)) / static_cast<double>(i - m);
-    }
-#pragma unroll
-    for (int j = -1; j <= 2; ++j) {
-      double aux = wr;
-#pragma unroll
-      for (int n = -1; n <= 2; ++n) {
-        if (n == j) continue;
-        aux = aux * (auxz - static_cast<double>(n)) / static_cast<double>(j - n);
+          for (int n = -1; n <= 2; ++n) {
+            if (n == j) continue;
+            aux = aux * (auxz - static_cast<double>(n)) / static_cast<double>(j - n);
+          }
+          w[i + 1][j + 1] = aux;
+        }
       }
-      w[i + 1][j + 1] = aux;
-    }
-  }
+      if (f < AF_N) {
+        double fa[4][4];
 #pragma unroll
-  for (int f = 0; f < AF_N; ++f) v[f] = 0.0;
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) fa[i][j] = at(i0 - 1 + i, j0 - 1 + j);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double2* r = rec(i0 - 1 + i, j0 - 1 + j);
-      const double wij = w[i][j];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int f = 0; f < AF_N / 2; ++f) {
-        const double2 t = r[f];
-        v[2 * f] = v[2 * f] + wij * t.x;
-        v[2 * f + 1] = v[2 * f + 1] + wij * t.y;
+          for (int j = 0; j < 4; ++j) out = out + w[i][j] * fa[i][j];
       }
     }
+  }
+  if (f < AF_N) sv[f] = out;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < AF_N; ++q) v[q] = sv[q];
+  __syncthreads();
 }
 
 // ------------------------------------------------------------ ah_source
 // h'' (sdh) and the area element (sarea) at (h, h', theta) for a horizon
 // centred at zcenter (:1983-2752).  Returns the error flag.
 __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, double zcenter, double& sdh,
-                         double& sarea) {
+                         double& sarea, double* sv) {
   const AhLevel& L0 = c.L[0];
   const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
   const double smallpi = M_PI;
@@ -275,7 +274,7 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
     if (r0 > L.rmin + L.G.dr && r0 < L.rmax - L.G.dr && z0 > L.zmin + L.G.dz && z0 < L.zmax - L.G.dz) lev = ll;
   }
   double v[AF_N];
-  ah_interp(c.L[lev], c.interporder, r0, z0, v);
+  ah_interp(c.L[lev], c.interporder, r0, z0, v, sv);
 
   const double interp_psi4 = (v[AF_psi] * v[AF_psi]) * (v[AF_psi] * v[AF_psi]);
   double Dphi[3];                                                                // :2247-2249
@@ -415,7 +414,8 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
 // zinit.  The reference's "local extrapolation" (:672-677) writes the slot
 // after the accepted one, which the next step overwrites before reading, so
 // it has no effect and is not restated.
-__device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, double* th_out, double* rad_out) {
+__device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, double* th_out, double* rad_out,
+                                double* sv) {
   const double zero = 0.0, half = 0.5, two = 2.0;
   const double smallpi = M_PI;
   double rad = std::fabs(zinit - zcenter);
@@ -432,7 +432,8 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
   if (c.eqsym && zcenter == zero) thetafinal = half * smallpi;
   else if (zcenter >= zero) thetafinal = smallpi - 1.0e-3;
   else thetafinal = 1.0e-3;
-  if (th_out) {
+  const bool rec = th_out != nullptr && threadIdx.x == 0;
+  if (rec) {
     th_out[0] = theta;
     rad_out[0] = rad;
   }
@@ -451,32 +452,32 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
     }
     const double dth2 = half * dth;
     double k_1, k_2, k_3, k_4, l_1, l_2, l_3, l_4, m_1, m_2, m_3, m_4;
-    if ((err = ah_source(c, rad, drad, theta, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad, drad, theta, zcenter, sdh, sarea, sv))) break;
     k_1 = dth * drad;
     l_1 = dth * sdh;
     m_1 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth, zcenter, sdh, sarea, sv))) break;
     k_2 = dth * (drad + 0.5 * l_1);
     l_2 = dth * sdh;
-    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth, zcenter, sdh, sarea, sv))) break;
     k_3 = dth * (drad + 0.5 * l_2);
     l_3 = dth * sdh;
-    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth, zcenter, sdh, sarea, sv))) break;
     k_4 = dth * (drad + l_3);
     l_4 = dth * sdh;
     const double drad_aux = drad + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
 
     k_1 = half * k_1;
     l_1 = half * l_1;
-    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth2, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth2, zcenter, sdh, sarea, sv))) break;
     k_2 = dth2 * (drad + 0.5 * l_1);
     l_2 = dth2 * sdh;
     m_2 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth2, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth2, zcenter, sdh, sarea, sv))) break;
     k_3 = dth2 * (drad + 0.5 * l_2);
     l_3 = dth2 * sdh;
     m_3 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth2, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth2, zcenter, sdh, sarea, sv))) break;
     k_4 = dth2 * (drad + l_3);
     l_4 = dth2 * sdh;
     m_4 = std::fabs(dth2) * sarea;
@@ -484,19 +485,19 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
     const double drad1 = drad + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
     const double area1 = area + sixth * (m_1 + 2.0 * m_2 + 2.0 * m_3 + m_4);
 
-    if ((err = ah_source(c, rad1, drad1, theta + dth, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad1, drad1, theta + dth, zcenter, sdh, sarea, sv))) break;
     k_1 = dth2 * drad1;
     l_1 = dth2 * sdh;
     m_1 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad1 + 0.5 * k_1, drad1 + 0.5 * l_1, theta + 1.5 * dth2, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad1 + 0.5 * k_1, drad1 + 0.5 * l_1, theta + 1.5 * dth2, zcenter, sdh, sarea, sv))) break;
     k_2 = dth2 * (drad1 + 0.5 * l_1);
     l_2 = dth2 * sdh;
     m_2 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad1 + 0.5 * k_2, drad1 + 0.5 * l_2, theta + 1.5 * dth2, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad1 + 0.5 * k_2, drad1 + 0.5 * l_2, theta + 1.5 * dth2, zcenter, sdh, sarea, sv))) break;
     k_3 = dth2 * (drad1 + 0.5 * l_2);
     l_3 = dth2 * sdh;
     m_3 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad1 + k_3, drad1 + l_3, theta + dth, zcenter, sdh, sarea))) break;
+    if ((err = ah_source(c, rad1 + k_3, drad1 + l_3, theta + dth, zcenter, sdh, sarea, sv))) break;
     k_4 = dth2 * (drad1 + l_3);
     l_4 = dth2 * sdh;
     m_4 = std::fabs(dth2) * sarea;
@@ -514,7 +515,7 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
       drad = drad2;
       area = area2;
       i = i + 1;
-      if (th_out) {
+      if (rec) {
         th_out[i] = theta;
         rad_out[i] = rad;
       }
@@ -533,12 +534,17 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
   return o;
 }
 
+// One shot per workgroup of one wave (kShotBlock = 64 lanes): every lane
+// runs the same integration on the same values (the control flow is
+// uniform), the lanes share the loads of each interpolation.
 __global__ void __launch_bounds__(kShotBlock) k_ah_shoot(const ShotIn* __restrict__ in, ShotOut* __restrict__ out,
                                                          int n) {
-  const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  __shared__ double sv[AF_N];
+  const int t = static_cast<int>(blockIdx.x);
   if (t >= n) return;
   const ShotIn s = in[t];
-  out[t] = ah_integrate(*s.ctx, s.zinit, s.zcenter, s.theta, s.rad);
+  const ShotOut o = ah_integrate(*s.ctx, s.zinit, s.zcenter, s.theta, s.rad, sv);
+  if (threadIdx.x == 0) out[t] = o;
 }
 
 // ================================================================ host side
@@ -631,8 +637,7 @@ int run_shots(hipStream_t st, AhState& buf, const std::vector<ShotIn>& in, std::
   AH_RC(ensure_capacity(buf, n));
   std::memcpy(buf.h_in, in.data(), sizeof(ShotIn) * n);
   AH_HIP(hipMemcpyAsync(buf.d_in, buf.h_in, sizeof(ShotIn) * n, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_ah_shoot, dim3((n + kShotBlock - 1) / kShotBlock), dim3(kShotBlock), 0, st, buf.d_in, buf.d_out,
-                     n);
+  hipLaunchKernelGGL(k_ah_shoot, dim3(n), dim3(kShotBlock), 0, st, buf.d_in, buf.d_out, n);
   AH_HIP(hipGetLastError());
   AH_HIP(hipMemcpyAsync(buf.h_out, buf.d_out, sizeof(ShotOut) * n, hipMemcpyDeviceToHost, st));
   AH_HIP(hipStreamSynchronize(st));
@@ -656,7 +661,7 @@ struct Search {
 // Walk the reference's control flow from the start.  Returns true when the
 // search is complete; otherwise `want` holds the shots to integrate next
 // (the missing one first, then its speculative continuation).
-bool walk(Search& s, std::vector<double>& want) {
+bool walk(Search& s, int depth, std::vector<double>& want) {
   const AhCtx& c = *s.c;
   const AhLevel& L0 = c.L[0];
   const double zero = 0.0, half = 0.5, two = 2.0;
@@ -697,10 +702,10 @@ bool walk(Search& s, std::vector<double>& want) {
             z = up ? z - half * dzl : z + half * dzl;
           }
         } else {
-          // the bisection: every node of the next kAhDepth steps; node
+          // the bisection: every node of the next `depth` steps; node
           // (a, b) shoots half*(a+b) and continues in (a, m) or (m, b)
           std::vector<std::pair<double, double>> level{{za, zb}};
-          for (int d = 0; d < kAhDepth; ++d) {
+          for (int d = 0; d < depth; ++d) {
             std::vector<std::pair<double, double>> next;
             for (const auto& ab : level) {
               const double m = half * (ab.first + ab.second);
@@ -922,11 +927,15 @@ int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
   for (;;) {
     in.clear();
     owner.clear();
+    int live = 0;
+    for (const Job& j : jobs) live += j.done ? 0 : 1;
+    int depth = 1;
+    while (depth < kAhMaxDepth && live * ((2 << depth) - 1) <= kAhResident) ++depth;
     for (size_t q = 0; q < jobs.size(); ++q) {
       Job& j = jobs[q];
       if (j.done) continue;
       j.s.zinit = j.zstart;
-      if (walk(j.s, want)) {
+      if (walk(j.s, depth, want)) {
         j.done = true;
         continue;
       }
