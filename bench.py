@@ -270,7 +270,8 @@ def main():
     ap.add_argument("--steps", type=int, default=8, help="timed coarse steps")
     ap.add_argument("--warmup", type=int, default=2, help="untimed coarse steps")
     ap.add_argument("--workload", default="batched_scalar",
-                    help="batched_scalar (default), serving, brill, plain_small, plain_256x512, plain_512x1024")
+                    help="batched_scalar (default), serving, brill, plain_small, plain_256x512, plain_512x1024, "
+                         "horizon (one apparent-horizon search only)")
     ap.add_argument("--members", type=int, default=64, help="spacetimes of the batched workload")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU time of the baseline sample")
@@ -297,6 +298,10 @@ def main():
 
     if args.workload == "batched_scalar":
         run_batched(args)
+        return
+    if args.workload == "horizon":
+        # the apparent-horizon search alone (for profiling the finder)
+        print(json.dumps({"horizon": horizon_probe(not args.no_cpu_baseline)}))
         return
     if args.workload == "serving":
         # the full serving mix (grids up to 512x1024, 200-2000 steps scaled by --serving-steps-scale)
