@@ -259,6 +259,18 @@ int oa_constraints(oa_handle* h, int box, int level);
  * reference's serial loop, so nm2 agrees to rounding).  A NaN in the range
  * returns OA_ERR_STATE (the reference stops the run there). */
 int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out);
+/* Checkpoint and restart (checkpointsave / checkpointload,
+ * src/base/checkpoint.f90:2-690).  The reference writes every evolved array,
+ * the fine-boundary time levels var_bound_* with the clocks t, t1, t2 of each
+ * level, and the par file, as text.  Here the same content, plus the
+ * per-level state of the auxiliary pass, is one opaque byte block.  A handle
+ * created with the same parameters and loaded from it continues bit for bit
+ * as the handle that saved it.  Between coarse steps only (OA_ERR_STATE
+ * inside a level's step); load refuses a block of other parameters. */
+int oa_checkpoint_size(const oa_handle* h, size_t* bytes);
+int oa_checkpoint_save(oa_handle* h, void* buf, size_t bytes);
+int oa_checkpoint_load(oa_handle* h, const void* buf, size_t bytes);
+
 /* One apparent-horizon search: what ahshoot (src/geometry/horizon_finder.f90:
  * 246-1103) finds for horizon k and writes to its ah_area / ah_mass files. */
 typedef struct oa_horizon {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -679,6 +680,144 @@ extern "C" int oa_params_footprint(const oa_params* pin, double* bytes) {
   }
   if (p.ahfind) total += ah_fixed_bytes(p.N_ah < 1 ? 1 : (p.N_ah > 3 ? 3 : p.N_ah));
   *bytes = total;
+  return OA_OK;
+}
+
+// ------------------------------------------------------------ checkpoint
+// Snapshot layout: a header (magic, version, level count, field count, and
+// per level the plane and strip sizes that a matching handle must have),
+// then per level the clocks, the flags and the device data: the current
+// state slab, the three fine-boundary time levels in time order and, with
+// the Gamma-driver, the pre-initialisation dtbeta planes.
+namespace {
+
+constexpr uint64_t kCkptMagic = 0x3154504b43584f41ull;   // "AOXCKPT1"
+
+struct CkptLevel {
+  double t, t1, t2;
+  int32_t s, dirty, phi_valid, aux_phi, dtb_pre_valid, pad;
+};
+
+size_t ckpt_level_bytes(const oa_handle* h, const Level& L) {
+  const size_t slab = sizeof(double) * static_cast<size_t>(h->nf) * L.G.plane;
+  const size_t strips = sizeof(double) * static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
+  const size_t pre = h->shift ? sizeof(double) * 2 * static_cast<size_t>(L.G.plane) : 0;
+  return sizeof(CkptLevel) + slab + 3 * strips + pre;
+}
+
+size_t ckpt_header_bytes(const oa_handle* h) { return sizeof(uint64_t) * (4 + 3 * static_cast<size_t>(h->nlev)); }
+
+int ckpt_check(const oa_handle* h, const char* who) {
+  for (const Level& L : h->L)
+    if (L.in_step) {
+      oa_set_error(std::string(who) + ": a level is inside a step (checkpoint between coarse steps)");
+      return OA_ERR_STATE;
+    }
+  return OA_OK;
+}
+
+}  // namespace
+
+extern "C" int oa_checkpoint_size(const oa_handle* h, size_t* bytes) {
+  if (!h || !bytes) return OA_ERR_ARG;
+  size_t n = ckpt_header_bytes(h);
+  for (const Level& L : h->L) n += ckpt_level_bytes(h, L);
+  *bytes = n;
+  return OA_OK;
+}
+
+extern "C" int oa_checkpoint_save(oa_handle* h, void* buf, size_t bytes) {
+  if (!h || !buf) return OA_ERR_ARG;
+  RC(ckpt_check(h, "oa_checkpoint_save"));
+  size_t need = 0;
+  RC(oa_checkpoint_size(h, &need));
+  if (bytes < need) {
+    oa_set_error("oa_checkpoint_save: buffer smaller than oa_checkpoint_size");
+    return OA_ERR_ARG;
+  }
+  char* p = static_cast<char*>(buf);
+  uint64_t* hd = reinterpret_cast<uint64_t*>(p);
+  hd[0] = kCkptMagic;
+  hd[1] = 1;
+  hd[2] = static_cast<uint64_t>(h->nlev);
+  hd[3] = static_cast<uint64_t>(h->nf);
+  for (int l = 0; l < h->nlev; ++l) {
+    const Level& L = h->L[l];
+    hd[4 + 3 * l] = static_cast<uint64_t>(L.G.plane);
+    hd[5 + 3 * l] = static_cast<uint64_t>(L.strip_r);
+    hd[6 + 3 * l] = static_cast<uint64_t>(L.strip_z);
+  }
+  p += ckpt_header_bytes(h);
+  for (Level& L : h->L) {
+    CkptLevel c{L.t, L.t1, L.t2, L.s, L.dirty ? 1 : 0, L.phi_valid ? 1 : 0, L.aux_phi ? 1 : 0,
+                L.dtb_pre_valid ? 1 : 0, 0};
+    std::memcpy(p, &c, sizeof c);
+    p += sizeof c;
+    const size_t slab = sizeof(double) * static_cast<size_t>(h->nf) * L.G.plane;
+    HIPCHK(hipMemcpyAsync(p, L.slab[L.cur], slab, hipMemcpyDeviceToHost, h->stream));
+    p += slab;
+    const size_t strips = sizeof(double) * static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
+    for (int tl = 0; tl < 3; ++tl) {
+      HIPCHK(hipMemcpyAsync(p, L.strip[L.ring[tl]], strips, hipMemcpyDeviceToHost, h->stream));
+      p += strips;
+    }
+    if (h->shift) {
+      const size_t pre = sizeof(double) * 2 * static_cast<size_t>(L.G.plane);
+      HIPCHK(hipMemcpyAsync(p, L.dtb_pre, pre, hipMemcpyDeviceToHost, h->stream));
+      p += pre;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return OA_OK;
+}
+
+extern "C" int oa_checkpoint_load(oa_handle* h, const void* buf, size_t bytes) {
+  if (!h || !buf) return OA_ERR_ARG;
+  RC(ckpt_check(h, "oa_checkpoint_load"));
+  size_t need = 0;
+  RC(oa_checkpoint_size(h, &need));
+  const char* p = static_cast<const char*>(buf);
+  const uint64_t* hd = reinterpret_cast<const uint64_t*>(p);
+  bool ok = bytes >= need && hd[0] == kCkptMagic && hd[1] == 1 && hd[2] == static_cast<uint64_t>(h->nlev) &&
+            hd[3] == static_cast<uint64_t>(h->nf);
+  for (int l = 0; ok && l < h->nlev; ++l) {
+    const Level& L = h->L[l];
+    ok = hd[4 + 3 * l] == static_cast<uint64_t>(L.G.plane) && hd[5 + 3 * l] == static_cast<uint64_t>(L.strip_r) &&
+         hd[6 + 3 * l] == static_cast<uint64_t>(L.strip_z);
+  }
+  if (!ok) {
+    oa_set_error("oa_checkpoint_load: the snapshot does not belong to a spacetime with these parameters");
+    return OA_ERR_ARG;
+  }
+  p += ckpt_header_bytes(h);
+  for (Level& L : h->L) {
+    CkptLevel c;
+    std::memcpy(&c, p, sizeof c);
+    p += sizeof c;
+    const size_t slab = sizeof(double) * static_cast<size_t>(h->nf) * L.G.plane;
+    HIPCHK(hipMemcpyAsync(L.slab[L.cur], p, slab, hipMemcpyHostToDevice, h->stream));
+    p += slab;
+    const size_t strips = sizeof(double) * static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
+    for (int tl = 0; tl < 3; ++tl) {
+      HIPCHK(hipMemcpyAsync(L.strip[L.ring[tl]], p, strips, hipMemcpyHostToDevice, h->stream));
+      p += strips;
+    }
+    if (h->shift) {
+      const size_t pre = sizeof(double) * 2 * static_cast<size_t>(L.G.plane);
+      HIPCHK(hipMemcpyAsync(L.dtb_pre, p, pre, hipMemcpyHostToDevice, h->stream));
+      p += pre;
+    }
+    L.t = c.t;
+    L.t1 = c.t1;
+    L.t2 = c.t2;
+    L.s = c.s;
+    L.dirty = c.dirty != 0;
+    L.phi_valid = c.phi_valid != 0;
+    L.aux_phi = c.aux_phi != 0;
+    L.dtb_pre_valid = c.dtb_pre_valid != 0;
+    L.cons_valid = false;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
   return OA_OK;
 }
 

@@ -27,7 +27,8 @@ module oaxis_mod
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
             oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule, &
-            oa_constraints, oa_ahfind, oa_horizon_shape, oa_batch_ahfind
+            oa_constraints, oa_ahfind, oa_horizon_shape, oa_batch_ahfind, &
+            oa_checkpoint_size, oa_checkpoint_save, oa_checkpoint_load
 
   ! result of one apparent-horizon search (oa_horizon in include/oaxis.h)
   type, bind(C), public :: oa_horizon
@@ -242,6 +243,22 @@ module oaxis_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int), value :: box, level
+    end function
+    ! checkpointsave / checkpointload: the whole state as one byte block
+    integer(c_int) function oa_checkpoint_size(h, bytes) bind(C, name="oa_checkpoint_size")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: h
+      integer(c_size_t) :: bytes
+    end function
+    integer(c_int) function oa_checkpoint_save(h, buf, bytes) bind(C, name="oa_checkpoint_save")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: h, buf
+      integer(c_size_t), value :: bytes
+    end function
+    integer(c_int) function oa_checkpoint_load(h, buf, bytes) bind(C, name="oa_checkpoint_load")
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: h, buf
+      integer(c_size_t), value :: bytes
     end function
     ! horizon_finder (shooting) for horizon k on the current state
     integer(c_int) function oa_ahfind(h, k, res) bind(C, name="oa_ahfind")

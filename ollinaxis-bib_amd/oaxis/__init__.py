@@ -196,6 +196,22 @@ class Spacetime:
         mom_z, CDelta_r, CDelta_z can be downloaded or reduced by name."""
         _check(lib().oa_constraints(self._h, 0, level), "oa_constraints")
 
+    def checkpoint(self):
+        """The whole evolution state as bytes (oa_checkpoint_save): every
+        level's evolved arrays, fine-boundary time levels, clocks and
+        auxiliary-pass flags.  Between coarse steps."""
+        n = ctypes.c_size_t()
+        _check(lib().oa_checkpoint_size(self._h, ctypes.byref(n)), "oa_checkpoint_size")
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().oa_checkpoint_save(self._h, buf, n.value), "oa_checkpoint_save")
+        return buf.raw
+
+    def restore(self, data):
+        """Load a checkpoint of a spacetime with the same parameters; the
+        evolution continues bit for bit as the one that saved it."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(lib().oa_checkpoint_load(self._h, buf, len(data)), "oa_checkpoint_load")
+
     def ahfind(self, k=0, shape=False):
         """horizon_finder (shooting) for horizon k on the current state:
         dict(found, ntheta, shots, errorflag, zcenter, zinit, dradlast, area,
@@ -350,6 +366,42 @@ class Batch:
 BASE_FIELDS = ["alpha", "chi", "phi", "A", "B", "H", "C", "trK", "KTA", "KTB", "KTC", "Delta_r", "Delta_z"]
 SHIFT_FIELDS = ["beta_r", "beta_z", "dtbeta_r", "dtbeta_z"]
 SCALAR_FIELDS = ["scalar_phi", "scalar_xi_r", "scalar_xi_z", "scalar_pi"]
+
+
+_CKPT_MAGIC = b"OAXCKPTF"
+
+
+def save_checkpoint(st, path):
+    """Write the spacetime's parameters and its whole state (Spacetime.
+    checkpoint) to one file: the restart file of checkpointsave."""
+    data = st.checkpoint()
+    raw = bytes(st.params)
+    with open(path, "wb") as f:
+        f.write(_CKPT_MAGIC)
+        f.write(len(raw).to_bytes(8, "little"))
+        f.write(raw)
+        f.write(data)
+
+
+def load_checkpoint(path):
+    """A new Spacetime from a file save_checkpoint wrote (checkpointload):
+    same parameters, same state; it continues bit for bit."""
+    with open(path, "rb") as f:
+        if f.read(8) != _CKPT_MAGIC:
+            raise OaxisError("%s is not a checkpoint of this library" % path)
+        n = int.from_bytes(f.read(8), "little")
+        raw = f.read(n)
+        data = f.read()
+    if n != ctypes.sizeof(Params):
+        raise OaxisError("%s was written with another parameter layout" % path)
+    p = Params.from_buffer_copy(raw)
+    st = Spacetime(p)
+    try:
+        st.restore(data)
+    except Exception:
+        st.close()
+        raise
+    return st
 
 
 def state_fields(params):
