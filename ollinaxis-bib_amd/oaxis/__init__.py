@@ -130,7 +130,8 @@ class Spacetime:
         dr, dz, dt = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(lib().oa_level_info(self._h, 0, level, ctypes.byref(Nr), ctypes.byref(Nz), ctypes.byref(g),
                                    ctypes.byref(dr), ctypes.byref(dz), ctypes.byref(dt)), "oa_level_info")
-        return dict(Nr=Nr.value, Nz=Nz.value, ghost=g.value, dr=dr.value, dz=dz.value, dt=dt.value)
+        return dict(Nr=Nr.value, Nz=Nz.value, ghost=g.value, dr=dr.value, dz=dz.value, dt=dt.value,
+                    eqsym=bool(self.params.eqsym))
 
     def shape(self, level=0):
         i = self.level_info(level)

@@ -59,3 +59,27 @@ def test_snapshot_of_other_parameters_is_refused(oaxis_mod):
         data = a.checkpoint()
         with pytest.raises(oaxis.OaxisError):
             b.restore(data)
+
+
+def test_output_writers_on_device_state(oaxis_mod, tmp_path):
+    """save0D/save1D/save2D on a device spacetime: the .2D values are the
+    downloaded array (15 digits) and the .tl norms the device reduction."""
+    import os
+    from oaxis import output
+    oaxis = oaxis_mod
+    p = oaxis.params_from_text(CASES["schwarzschild_2levels_rk4"][0])
+    with oaxis.Spacetime(p) as st:
+        st.initial_data()
+        st.initial_aux()
+        st.evolve(1)
+        out = str(tmp_path)
+        for lev in range(st.nlevels):
+            output.save0D(st, out, ["alpha", "trK"], lev, new=True)
+            output.save1D(st, out, ["alpha"], lev, new=True)
+            output.save2D(st, out, ["alpha"], lev, new=True)
+        a = st.download("alpha", 1)
+        rows = open(os.path.join(out, "alpha_b0l1.2D")).read().split("\n")
+        vals = [float(l[46:]) for l in rows[1:1 + a.shape[0]]]
+        assert np.allclose(vals, a[:, 0] + 1e-30, rtol=1e-14, atol=0)
+        nm2 = float(open(os.path.join(out, "trK_b0l0_nm2.tl")).read().split("\n")[1][20:])
+        assert abs(nm2 - st.norms0d("trK", 0)["nm2"]) <= 1e-8 * abs(nm2)
