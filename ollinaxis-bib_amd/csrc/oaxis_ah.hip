@@ -190,7 +190,10 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
   const double deltaz = z0 - (static_cast<double>(j0) - G.zoff) * G.dz;
   auto ci = [&](int i) { return i < 1 - g ? 1 - g : (i > G.Nr ? G.Nr : i); };
   auto cj = [&](int j) { return j < 1 - g ? 1 - g : (j > G.Nz ? G.Nz : j); };
-  const int f = static_cast<int>(threadIdx.x);
+  // each half-wave evaluates its own point (see ah_integrate); lane f of a
+  // half-wave sums quantity f
+  const int f = static_cast<int>(threadIdx.x & 31u);
+  double* slot = sv + (threadIdx.x >> 5) * AF_N;
   // quantity f of one point record
   auto at = [&](int i, int j) { return L.q[IDX(G, ci(i), cj(j)) * AF_N + f]; };
   double out = 0.0;
@@ -238,10 +241,10 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
       }
     }
   }
-  if (f < AF_N) sv[f] = out;
+  if (f < AF_N) slot[f] = out;
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < AF_N; ++q) v[q] = sv[q];
+  for (int q = 0; q < AF_N; ++q) v[q] = slot[q];
   __syncthreads();
 }
 
@@ -254,19 +257,26 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
   const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
   const double smallpi = M_PI;
   sdh = 0.0;
-  if (rad < 0.0) return 1;                                                       // :2140-2145
+  // The checks that end the reference's ah_source early set err here.  Every
+  // lane still runs one interpolation (at a harmless point when err is set)
+  // so that both half-waves of a shot always meet its barriers together.
+  int err = (rad < 0.0) ? 1 : 0;                                                 // :2140-2145
   const double rc = rad * std::sin(th);                                          // :2151-2159
   const double zc = rad * std::cos(th);
-  const double r0 = rc;
+  double r0 = rc;
   double z0 = zc + zcenter;
   if (c.eqsym && z0 < 0.0) z0 = -z0;
   // a non-finite position (never produced by a finite integration) is refused
   // before it could index the grid
-  if (!(std::isfinite(r0) && std::isfinite(z0))) return 1;
-  if (r0 > c.ahrmax * L0.rmax || std::fabs(z0) > c.ahrmax * L0.zmax) return 1;  // :2163-2168
+  if (!(std::isfinite(r0) && std::isfinite(z0))) err = 1;
+  if (r0 > c.ahrmax * L0.rmax || std::fabs(z0) > c.ahrmax * L0.zmax) err = 1;   // :2163-2168
   // :2172-2178 (the z test against zmax subtracts drl, as the reference does)
-  if (r0 < L0.rmin + L0.G.dr || r0 > L0.rmax - L0.G.dr || z0 < L0.zmin + L0.G.dz || z0 > L0.zmax - L0.G.dr) return 1;
-  if (z0 * zcenter < 0.0 && c.method == OA_AH_SHOOT) return 1;                  // :2185-2190
+  if (r0 < L0.rmin + L0.G.dr || r0 > L0.rmax - L0.G.dr || z0 < L0.zmin + L0.G.dz || z0 > L0.zmax - L0.G.dr) err = 1;
+  if (z0 * zcenter < 0.0 && c.method == OA_AH_SHOOT) err = 1;                   // :2185-2190
+  if (err) {
+    r0 = 0.5 * (L0.rmin + L0.rmax);
+    z0 = 0.5 * (L0.zmin + L0.zmax);
+  }
 
   int lev = 0;                                                                   // :2199-2213
   for (int ll = 1; ll < c.nlev; ++ll) {
@@ -275,6 +285,7 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
   }
   double v[AF_N];
   ah_interp(c.L[lev], c.interporder, r0, z0, v, sv);
+  if (err) return 1;
 
   const double interp_psi4 = (v[AF_psi] * v[AF_psi]) * (v[AF_psi] * v[AF_psi]);
   double Dphi[3];                                                                // :2247-2249
@@ -415,7 +426,7 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
 // after the accepted one, which the next step overwrites before reading, so
 // it has no effect and is not restated.
 __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, double* th_out, double* rad_out,
-                                double* sv) {
+                                double* sv, double* xs, int* xe) {
   const double zero = 0.0, half = 0.5, two = 2.0;
   const double smallpi = M_PI;
   double rad = std::fabs(zinit - zcenter);
@@ -451,39 +462,56 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
       if (theta + dth < thetafinal) dth = thetafinal - theta;
     }
     const double dth2 = half * dth;
+    double rad1, drad1, area1, drad_aux;
     double k_1, k_2, k_3, k_4, l_1, l_2, l_3, l_4, m_1, m_2, m_3, m_4;
     if ((err = ah_source(c, rad, drad, theta, zcenter, sdh, sarea, sv))) break;
     k_1 = dth * drad;
     l_1 = dth * sdh;
     m_1 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth, zcenter, sdh, sarea, sv))) break;
-    k_2 = dth * (drad + 0.5 * l_1);
-    l_2 = dth * sdh;
-    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth, zcenter, sdh, sarea, sv))) break;
-    k_3 = dth * (drad + 0.5 * l_2);
-    l_3 = dth * sdh;
-    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth, zcenter, sdh, sarea, sv))) break;
-    k_4 = dth * (drad + l_3);
-    l_4 = dth * sdh;
-    const double drad_aux = drad + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
-
-    k_1 = half * k_1;
-    l_1 = half * l_1;
-    if ((err = ah_source(c, rad + 0.5 * k_1, drad + 0.5 * l_1, theta + 0.5 * dth2, zcenter, sdh, sarea, sv))) break;
-    k_2 = dth2 * (drad + 0.5 * l_1);
-    l_2 = dth2 * sdh;
-    m_2 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * dth2, zcenter, sdh, sarea, sv))) break;
-    k_3 = dth2 * (drad + 0.5 * l_2);
-    l_3 = dth2 * sdh;
-    m_3 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad + k_3, drad + l_3, theta + dth2, zcenter, sdh, sarea, sv))) break;
-    k_4 = dth2 * (drad + l_3);
-    l_4 = dth2 * sdh;
-    m_4 = std::fabs(dth2) * sarea;
-    const double rad1 = rad + sixth * (k_1 + 2.0 * k_2 + 2.0 * k_3 + k_4);
-    const double drad1 = drad + sixth * (l_1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
-    const double area1 = area + sixth * (m_1 + 2.0 * m_2 + 2.0 * m_3 + m_4);
+    // The large step's last three stages and the first small step's last
+    // three are independent: the upper half-wave runs the large step, the
+    // lower half the small one, in one instruction stream (step h, first
+    // stage k1/l1 per half, in the reference's expressions).  An error in
+    // either ends the step as the reference's first error would: nothing of
+    // the step is kept.
+    {
+      const bool big = (threadIdx.x >> 5) != 0;
+      const double h = big ? dth : dth2;
+      const double k1 = big ? k_1 : half * k_1;
+      const double l1 = big ? l_1 : half * l_1;
+      int e = ah_source(c, rad + 0.5 * k1, drad + 0.5 * l1, theta + 0.5 * h, zcenter, sdh, sarea, sv);
+      k_2 = h * (drad + 0.5 * l1);
+      l_2 = h * sdh;
+      m_2 = std::fabs(dth2) * sarea;
+      e |= ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * h, zcenter, sdh, sarea, sv);
+      k_3 = h * (drad + 0.5 * l_2);
+      l_3 = h * sdh;
+      m_3 = std::fabs(dth2) * sarea;
+      e |= ah_source(c, rad + k_3, drad + l_3, theta + h, zcenter, sdh, sarea, sv);
+      k_4 = h * (drad + l_3);
+      l_4 = h * sdh;
+      m_4 = std::fabs(dth2) * sarea;
+      // exchange: the small step's (rad1, drad1, area1), the large step's
+      // drad_aux and both error flags reach every lane
+      if (threadIdx.x == 0) {
+        xs[0] = rad + sixth * (k1 + 2.0 * k_2 + 2.0 * k_3 + k_4);
+        xs[1] = drad + sixth * (l1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
+        xs[2] = area + sixth * (m_1 + 2.0 * m_2 + 2.0 * m_3 + m_4);
+        xe[0] = e;
+      }
+      if (threadIdx.x == 32) {
+        xs[3] = drad + sixth * (l1 + 2.0 * l_2 + 2.0 * l_3 + l_4);
+        xe[1] = e;
+      }
+      __syncthreads();
+      err = xe[0] | xe[1];
+      rad1 = xs[0];
+      drad1 = xs[1];
+      area1 = xs[2];
+      drad_aux = xs[3];
+      __syncthreads();
+    }
+    if (err) break;
 
     if ((err = ah_source(c, rad1, drad1, theta + dth, zcenter, sdh, sarea, sv))) break;
     k_1 = dth2 * drad1;
@@ -534,16 +562,20 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
   return o;
 }
 
-// One shot per workgroup of one wave (kShotBlock = 64 lanes): every lane
-// runs the same integration on the same values (the control flow is
-// uniform), the lanes share the loads of each interpolation.
+// One shot per workgroup of one wave (kShotBlock = 64 lanes): the lanes
+// share the loads of each interpolation, and the two halves of the wave run
+// the independent stages of a step side by side (8 sequential source
+// evaluations per step instead of 11); elsewhere every lane runs the same
+// integration on the same values.
 __global__ void __launch_bounds__(kShotBlock) k_ah_shoot(const ShotIn* __restrict__ in, ShotOut* __restrict__ out,
                                                          int n) {
-  __shared__ double sv[AF_N];
+  __shared__ double sv[2 * AF_N];
+  __shared__ double xs[4];
+  __shared__ int xe[2];
   const int t = static_cast<int>(blockIdx.x);
   if (t >= n) return;
   const ShotIn s = in[t];
-  const ShotOut o = ah_integrate(*s.ctx, s.zinit, s.zcenter, s.theta, s.rad, sv);
+  const ShotOut o = ah_integrate(*s.ctx, s.zinit, s.zcenter, s.theta, s.rad, sv, xs, xe);
   if (threadIdx.x == 0) out[t] = o;
 }
 

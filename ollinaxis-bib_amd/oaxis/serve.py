@@ -81,18 +81,40 @@ class Server:
         phi0 = st.norms0d("scalar_phi", 0)["nm1"] if scalar else 0.0
         ah = bool(p.ahfind)
         if ah:
-            # the search at t = 0 (evolve.f90:251): only a horizon that forms
-            # later retires the request
-            watch = not (p.ahafter <= 0.0 and st.ahfind(0)["found"])
+            # the search at t = 0 (evolve.f90:251) runs with the other new
+            # members' before the next step (see run); only a horizon that
+            # forms later retires the request
+            watch = True
         else:
             # only a collapse that forms during the evolution retires a
             # request (a puncture's lapse starts near zero)
             watch = st.norms0d("alpha", st.nlevels - 1)["min"] >= self.alpha_collapse
         batch.add(st)
         active.append(dict(req=req, st=st, rec=rec, steps=0, scalar=scalar, phi0=phi0, watch=watch, ah=ah,
+                           ah_t0=ah and p.ahafter <= 0.0,
                            ah_every=max(1, int(p.ahfind_every)), ah_after=float(p.ahafter),
                            dt0=st.level_info(0)["dt"], u0=st.point_updates()))
         return used + fp
+
+    def _initial_searches(self, batch, active):
+        """The t = 0 horizon searches of the members admitted since the last
+        step, as one lockstep batch: a horizon present at t = 0 does not
+        retire its request."""
+        new = [a for a in active if a.get("ah_t0")]
+        if not new:
+            return []
+        failed = []
+        try:
+            found = batch.ahfind([(a["st"], 0) for a in new])
+        except self.oaxis.OaxisError:
+            found = None
+        for q, a in enumerate(new):
+            a["ah_t0"] = False
+            if found is None:
+                failed.append(a)
+            elif found[q]["found"]:
+                a["watch"] = False
+        return failed
 
     def _retire(self, a, reason, now, batch):
         st, rec = a["st"], a["rec"]
@@ -151,6 +173,14 @@ class Server:
                     now = clock() - t0
                 if not active:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                    continue
+                for a in self._initial_searches(batch, active):
+                    rec = self._retire(a, "failed", clock() - t0, batch)
+                    total_updates += rec.point_updates
+                    used -= rec.footprint
+                    active.remove(a)
+                    done.append(rec)
+                if not active:
                     continue
                 batch.coarse_step()
                 for a in active:
