@@ -69,6 +69,29 @@ def measured_traffic(workload, algorithmic_bytes):
     return None
 
 
+def fp64_roofline(workload, points_per_launch, avg_launch_ms):
+    """FP64 rate of the RHS kernel: the counter-measured FP64 operations per
+    point of one RHS evaluation for this workload (committed summary,
+    SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64) times the points one launch
+    evaluates, over the launch's average duration; None without a
+    measurement."""
+    path = os.path.join(ROOT, "profiles", "k_rhs_fp64_valu.md")
+    per_point = None
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "flop_per_point" and parts[1] == workload:
+                    per_point = float(parts[2])
+    except OSError:
+        pass
+    if per_point is None or avg_launch_ms <= 0.0:
+        return None
+    tflops = per_point * points_per_launch / (avg_launch_ms * 1e-3) / 1e12
+    return {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_point": per_point}
+
+
 def cpu_baseline(texts, min_seconds):
     """The oracle (plain C restatement, one core) on a bounded sample: coarse
     steps of the given configurations in turn (one step each, cycling) until
@@ -248,6 +271,8 @@ def run_batched(args):
                      "kernel": "k_rhs_table", "avg_launch_ms": rhs_ms / rhs_n if rhs_n else 0.0,
                      "launches": rhs_n,
                      "algorithmic_bytes_per_launch": rhs_bytes / rhs_n if rhs_n else 0.0},
+        "fp64": fp64_roofline(args.workload, rhs_points / rhs_n if rhs_n else 0.0,
+                              rhs_ms / rhs_n if rhs_n else 0.0),
     }
     rec["horizon"] = horizon_probe(not args.no_cpu_baseline)
     if args.serving_requests > 0:
@@ -374,6 +399,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, bytes_per_launch),
                      "kernel": "k_rhs_table", "avg_launch_ms": rhs_ms, "launches": rhs_n,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        "fp64": fp64_roofline(args.workload, points_per_launch, rhs_ms),
     }
     st.close()
     if not args.no_cpu_baseline:
