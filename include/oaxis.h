@@ -300,7 +300,10 @@ typedef struct oa_horizon {
 int oa_ahfind(oa_handle* h, int k, oa_horizon* out);
 /* (theta_i, h(theta_i)), i = 0..ntheta, of the last integration of horizon k
  * (what ahshoot writes to ah_radius_SH, :956-966; h = 0 when not found).
- * cap is the room in theta and rad; *n receives ntheta + 1. */
+ * cap is the room in theta and rad; *n receives ntheta + 1 (also when the
+ * arrays are null, to size them).  The integration is run again on the
+ * quantities the search interpolated, so it needs no state; a later search
+ * of the same spacetime replaces them (OA_ERR_STATE: search again). */
 int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n);
 
 /* Device memory (bytes) oa_create allocates for these parameters: what a

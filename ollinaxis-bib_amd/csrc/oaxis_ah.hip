@@ -624,7 +624,13 @@ struct AhState {
   double* d_theta[3] = {nullptr, nullptr, nullptr};   // last integration of horizon k
   double* d_rad[3] = {nullptr, nullptr, nullptr};
   int found[3] = {0, 0, 0};
-  std::vector<double> theta[3], rad[3];
+  // The last integration of horizon k is recorded on demand (oa_horizon_shape)
+  // by integrating it again on the point records it was found on: gen counts
+  // rebuilds of the records, gen_of[k] is the one horizon k's search used.
+  long gen = 0;
+  long gen_of[3] = {-1, -1, -1};
+  double last_z[3] = {0.0, 0.0, 0.0}, last_zc[3] = {0.0, 0.0, 0.0};
+  int n_of[3] = {0, 0, 0};
 };
 
 namespace {
@@ -814,6 +820,7 @@ int build_fields(oa_handle* h, AhState& a) {
     AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.dctx), sizeof(AhCtx)));
   }
   const oa_params& p = h->p;
+  ++a.gen;
   AhCtx& c = a.hctx;
   std::memset(&c, 0, sizeof c);
   c.nlev = nlev;
@@ -920,8 +927,7 @@ struct Job {
 };
 
 // The searches of `jobs` in lockstep on stream st: every round integrates
-// the speculative shots of all unfinished searches in one launch.  Then the
-// last integration of each search is recorded (one more launch).
+// the speculative shots of all unfinished searches in one launch.
 int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
   for (Job& j : jobs) {
     AH_RC(ah_check(j.h, j.k));
@@ -980,33 +986,14 @@ int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
     AH_RC(run_shots(st, buf, in, res));
     for (size_t t = 0; t < in.size(); ++t) jobs[owner[t].first].s.table[key_of(owner[t].second)] = res[t];
   }
-  // the last integration of every search, recorded (ah_radius_SH, :956-966)
-  in.clear();
-  owner.clear();
-  for (size_t q = 0; q < jobs.size(); ++q) {
-    Job& j = jobs[q];
+  // where to integrate the last shot again when its points are asked for
+  for (Job& j : jobs) {
     AhState& a = *j.h->ah;
-    a.theta[j.k].clear();
-    a.rad[j.k].clear();
-    if (j.s.shots == 0) continue;
-    if (!a.d_theta[j.k]) {
-      AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_theta[j.k]), sizeof(double) * (kAhNthetaMax + 1)));
-      AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_rad[j.k]), sizeof(double) * (kAhNthetaMax + 1)));
-    }
-    in.push_back(ShotIn{a.dctx, j.s.last_zinit, j.s.zcenter, a.d_theta[j.k], a.d_rad[j.k]});
-    owner.push_back({q, j.s.last_zinit});
+    a.gen_of[j.k] = j.s.shots > 0 ? a.gen : -1;
+    a.last_z[j.k] = j.s.last_zinit;
+    a.last_zc[j.k] = j.s.zcenter;
+    a.n_of[j.k] = j.s.shots > 0 ? j.s.last.ntheta + 1 : 0;
   }
-  AH_RC(run_shots(st, buf, in, res));
-  for (size_t t = 0; t < in.size(); ++t) {
-    Job& j = jobs[owner[t].first];
-    AhState& a = *j.h->ah;
-    const int n = res[t].ntheta + 1;
-    a.theta[j.k].resize(static_cast<size_t>(n));
-    a.rad[j.k].resize(static_cast<size_t>(n));
-    AH_HIP(hipMemcpyAsync(a.theta[j.k].data(), a.d_theta[j.k], sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    AH_HIP(hipMemcpyAsync(a.rad[j.k].data(), a.d_rad[j.k], sizeof(double) * n, hipMemcpyDeviceToHost, st));
-  }
-  AH_HIP(hipStreamSynchronize(st));
 
   const double smallpi = M_PI, half = 0.5, two = 2.0;
   for (size_t q = 0; q < jobs.size(); ++q) {
@@ -1014,8 +1001,6 @@ int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
     const Search& s = j.s;
     AhState& a = *j.h->ah;
     a.found[j.k] = s.found;
-    if (!s.found)
-      for (double& v : a.rad[j.k]) v = 0.0;
     oa_horizon& o = out[q];
     std::memset(&o, 0, sizeof o);
     o.found = s.found;
@@ -1079,18 +1064,37 @@ extern "C" int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad,
     oa_set_error("oa_horizon_shape: bad argument");
     return OA_ERR_ARG;
   }
-  if (!h->ah || h->ah->theta[k].empty()) {
+  if (!h->ah || h->ah->n_of[k] == 0) {
     oa_set_error("oa_horizon_shape: no search of this horizon yet (call oa_ahfind)");
     return OA_ERR_STATE;
   }
-  const AhState& a = *h->ah;
-  const int m = static_cast<int>(a.theta[k].size());
+  AhState& a = *h->ah;
+  if (a.gen_of[k] != a.gen) {
+    oa_set_error("oa_horizon_shape: a later search rebuilt the horizon records; search this horizon again");
+    return OA_ERR_STATE;
+  }
+  const int m = a.n_of[k];
   *n = m;
   if (!theta || !rad || cap < m) {
     oa_set_error("oa_horizon_shape: arrays too small (need *n entries)");
     return OA_ERR_ARG;
   }
-  std::memcpy(theta, a.theta[k].data(), sizeof(double) * m);
-  std::memcpy(rad, a.rad[k].data(), sizeof(double) * m);
+  // the search's last integration again, recording its accepted points
+  if (!a.d_theta[k]) {
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_theta[k]), sizeof(double) * (kAhNthetaMax + 1)));
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_rad[k]), sizeof(double) * (kAhNthetaMax + 1)));
+  }
+  std::vector<ShotRes> res;
+  AH_RC(run_shots(h->stream, a, std::vector<ShotIn>{ShotIn{a.dctx, a.last_z[k], a.last_zc[k], a.d_theta[k], a.d_rad[k]}},
+                  res));
+  if (res[0].ntheta + 1 != m) {
+    oa_set_error("oa_horizon_shape: the repeated integration differs from the search's");
+    return OA_ERR_STATE;
+  }
+  AH_HIP(hipMemcpyAsync(theta, a.d_theta[k], sizeof(double) * m, hipMemcpyDeviceToHost, h->stream));
+  AH_HIP(hipMemcpyAsync(rad, a.d_rad[k], sizeof(double) * m, hipMemcpyDeviceToHost, h->stream));
+  AH_HIP(hipStreamSynchronize(h->stream));
+  if (!a.found[k])                                                           // :962-966
+    for (int i = 0; i < m; ++i) rad[i] = 0.0;
   return OA_OK;
 }
