@@ -218,3 +218,67 @@ def test_serving_retires_on_horizon(oaxis_mod):
     assert by[0].reason == "horizon" and by[0].steps == first, (by[0].reason, by[0].steps, first)
     assert by[0].horizon["found"] == 1 and abs(by[0].horizon["mass"] - 2.0) < 1e-3
     assert by[1].reason == "final_time" and by[1].steps == 6
+
+
+COLLAPSE = """
+mattertype = scalar
+scalarpotential = none
+idata = scalarpulse
+scalarpulse_type = shell
+scalar_a0 = 0.6
+scalar_r0 = 0.0
+scalar_z0 = 0.0
+scalar_sr0 = 1.0
+scalar_sz0 = 1.0
+slicing = 1+log
+gauge_f = 2.0
+shift = Gammadriver2
+drivercsi = 0.75
+integrator = rk4
+order = four
+eqsym = .true.
+Nrtotal = 32
+Nztotal = 32
+Nl0 = 1
+dr0 = 0.125
+dz0 = 0.125
+dtfac = 0.25
+geodiss = 0.1
+scalardiss = 0.1
+boundtype = radiative
+"""
+
+
+def test_scalar_collapse_forms_a_horizon(oaxis_mod, oracle_mod):
+    """A strong scalar pulse (Gamma-driver, 2 levels) collapses: searching
+    every 2 coarse steps, device and oracle agree on every search and find
+    the horizon at the same step; the serving loop retires the request
+    there with reason "horizon"."""
+    oaxis = oaxis_mod
+    p = oaxis.params_from_text(COLLAPSE)
+    orc = oracle_mod.Oracle(p)
+    orc.initial_data()
+    st = device_from_oracle(oaxis, orc, p)
+    orc.initial_aux()
+    st.initial_aux()
+    formed = None
+    for step in range(1, 25):
+        orc.coarse_step()
+        st.coarse_step()
+        if step % 2:
+            continue
+        dev = st.ahfind(0, shape=True)
+        ref = orc.ahfind(0, shape=True)
+        compare(dev, ref, "collapse step %d" % step)
+        if dev["found"]:
+            formed = (step, dev["mass"])
+            break
+    st.close()
+    orc.close()
+    print("collapse: horizon at coarse step %s" % (formed,))
+    assert formed is not None and 4 <= formed[0] <= 20 and 0.1 < formed[1] < 1.0
+    from oaxis.serve import Request, Server
+    records, _ = Server(1e12, check_every=0).run(
+        [Request(0, COLLAPSE + "ahfind = .true.\nahfind_every = 2\n", 40, 0.0, "collapse")])
+    assert records[0].reason == "horizon" and records[0].steps == formed[0]
+    assert abs(records[0].horizon["mass"] - formed[1]) < 1e-9
