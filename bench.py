@@ -316,7 +316,7 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (every level of a spacetime stays resident on "
+        print("bench.py: the library runs on one GPU only (every level of a spacetime stays resident on "
               "one device; batched instances shard by HBM footprint); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
