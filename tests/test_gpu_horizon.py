@@ -282,3 +282,21 @@ def test_scalar_collapse_forms_a_horizon(oaxis_mod, oracle_mod):
         [Request(0, COLLAPSE + "ahfind = .true.\nahfind_every = 2\n", 40, 0.0, "collapse")])
     assert records[0].reason == "horizon" and records[0].steps == formed[0]
     assert abs(records[0].horizon["mass"] - formed[1]) < 1e-9
+
+
+def test_critical_collapse_bisection(oaxis_mod):
+    """Two rounds of amplitude bisection of the collapsing pulse family
+    through the serving loop: outcomes are "no horizon" below and "horizon"
+    above a threshold, and the bracket shrinks by members - 1 per round."""
+    from oaxis.bisection import critical_bisection
+    res = critical_bisection(lambda a: COLLAPSE.replace("scalar_a0 = 0.6", "scalar_a0 = %r" % a),
+                             0.2, 0.8, members=7, rounds=2, steps=48, ahfind_every=2)
+    r1, r2 = res["rounds"]
+    print("bisection: %s -> [%.6f, %.6f], %.2f s" % ([(round(a, 3), h) for a, h in zip(r1["amplitudes"], r1["horizon"])],
+                                                    res["lo"], res["hi"], res["wall_s"]))
+    assert r1["horizon"] == sorted(r1["horizon"])                 # no horizon, then horizon
+    assert not r1["horizon"][0] and r1["horizon"][-1]
+    k = r1["horizon"].index(True)
+    assert r2["amplitudes"][0] == r1["amplitudes"][k - 1] and r2["amplitudes"][-1] == r1["amplitudes"][k]
+    assert abs((res["hi"] - res["lo"]) - 0.6 / 36) < 1e-12
+    assert res["lo"] < 0.6 and res["hi"] > 0.3
