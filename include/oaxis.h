@@ -295,8 +295,9 @@ typedef struct oa_horizon {
  * interpolates are formed on the device for every level, and the shots of
  * the search (the march along the axis and the bisection) are integrated on
  * the device, many per launch; the sequence of shots and every decision
- * between them are the reference's.  ahmove = .true. and the spectral
- * method return OA_ERR_UNSUPPORTED. */
+ * between them are the reference's.  With ahmove the centres of horizons 1
+ * and 2 first move to the nearest maximum of phi on the axis (:117-180).
+ * The spectral method returns OA_ERR_UNSUPPORTED. */
 int oa_ahfind(oa_handle* h, int k, oa_horizon* out);
 /* (theta_i, h(theta_i)), i = 0..ntheta, of the last integration of horizon k
  * (what ahshoot writes to ah_radius_SH, :956-966; h = 0 when not found).

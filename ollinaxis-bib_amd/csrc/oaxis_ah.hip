@@ -579,6 +579,16 @@ __global__ void __launch_bounds__(kShotBlock) k_ah_shoot(const ShotIn* __restric
   if (threadIdx.x == 0) out[t] = o;
 }
 
+// Dz_phi at one point of one level (the moving-centre walk of
+// horizon_finder.f90:117-180), by one wave as in the shots.
+__global__ void __launch_bounds__(kShotBlock) k_ah_dzphi(const AhCtx* __restrict__ c, int level, double r0,
+                                                         double z0, double* __restrict__ out) {
+  __shared__ double sv[2 * AF_N];
+  double v[AF_N];
+  ah_interp(c->L[level], c->interporder, r0, z0, v, sv);
+  if (threadIdx.x == 0) out[0] = v[AF_Dzphi];
+}
+
 // ================================================================ host side
 int hip_rc(hipError_t e, const char* what) {
   if (e == hipSuccess) return OA_OK;
@@ -631,6 +641,7 @@ struct AhState {
   long gen_of[3] = {-1, -1, -1};
   double last_z[3] = {0.0, 0.0, 0.0}, last_zc[3] = {0.0, 0.0, 0.0};
   int n_of[3] = {0, 0, 0};
+  double* d_scalar = nullptr;             // one value back from the device
 };
 
 namespace {
@@ -869,6 +880,7 @@ void ah_release(oa_handle* h) {
   for (double* p : a->planes)
     if (p) (void)hipFree(p);
   if (a->dctx) (void)hipFree(a->dctx);
+  if (a->d_scalar) (void)hipFree(a->d_scalar);
   for (int k = 0; k < 3; ++k) {
     if (a->d_theta[k]) (void)hipFree(a->d_theta[k]);
     if (a->d_rad[k]) (void)hipFree(a->d_rad[k]);
@@ -902,8 +914,8 @@ int ah_check(const oa_handle* h, int k) {
     oa_set_error("oa_ahfind: horizon 1 must be above the equator and horizon 2 below");
     return OA_ERR_PARAM;
   }
-  if (p.ahmove || p.ahmethod != OA_AH_SHOOT) {
-    oa_set_error("oa_ahfind: only ahmethod = shoot with ahmove = .false. is supported");
+  if (p.ahmethod != OA_AH_SHOOT) {
+    oa_set_error("oa_ahfind: only ahmethod = shoot is supported");
     return OA_ERR_UNSUPPORTED;
   }
   if (h->nlev > kAhMaxLevels) {
@@ -916,6 +928,42 @@ int ah_check(const oa_handle* h, int k) {
       return OA_ERR_STATE;
     }
   return OA_OK;
+}
+
+// Moving centre (horizon_finder.f90:117-180): from *z, step along the axis
+// by dz0 towards larger phi until Dz_phi changes sign; the centre is the
+// linear zero of Dz_phi.  Dz_phi is interpolated on the device, on the
+// finest level holding the starting point, one value per step.  A walk that
+// leaves that level returns OA_ERR_STATE (the reference stops in interp).
+int move_centre(oa_handle* h, AhState& a, double* z) {
+  const AhCtx& c = a.hctx;
+  const double r0 = 0.0;
+  double z0 = *z, z0_old = 0.0, Dzphi = 0.0, Dzphi_old = 0.0;
+  int level = 0;
+  for (int ll = 1; ll < c.nlev; ++ll) {
+    const AhLevel& L = c.L[ll];
+    if (r0 > L.rmin + L.G.dr && r0 < L.rmax - L.G.dr && z0 > L.zmin + L.G.dz && z0 < L.zmax - L.G.dz) level = ll;
+  }
+  const AhLevel& L = c.L[level];
+  if (!a.d_scalar) AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_scalar), sizeof(double)));
+  for (int it = 0;; ++it) {
+    if (it > 1000000 || z0 < L.zmin || z0 > L.zmax) {
+      oa_set_error("oa_ahfind: the moving-centre walk left the grid");
+      return OA_ERR_STATE;
+    }
+    hipLaunchKernelGGL(k_ah_dzphi, dim3(1), dim3(kShotBlock), 0, h->stream, a.dctx, level, r0, z0, a.d_scalar);
+    AH_HIP(hipGetLastError());
+    AH_HIP(hipMemcpyAsync(&Dzphi, a.d_scalar, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    AH_HIP(hipStreamSynchronize(h->stream));
+    if (Dzphi * Dzphi_old >= 0.0) {
+      z0_old = z0;
+      Dzphi_old = Dzphi;
+      z0 = (Dzphi > 0.0) ? z0 + h->p.dz0 : z0 - h->p.dz0;
+    } else {
+      *z = z0_old - Dzphi_old * (z0 - z0_old) / (Dzphi - Dzphi_old);
+      return OA_OK;
+    }
+  }
 }
 
 struct Job {
@@ -942,6 +990,7 @@ int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
       zah[1] = p.z_ah1;
       zah[2] = p.z_ah2;
     }
+    if (p.ahmove && p.N_ah > 1 && j.k > 0) AH_RC(move_centre(j.h, a, &zah[j.k]));
     const double zcenter = zah[j.k];
     const AhLevel& L0 = a.hctx.L[0];
     j.s.c = &a.hctx;

@@ -518,11 +518,36 @@ int or_ahfind(or_handle* H, int k, int* found, oa_horizon* out, double* theta_ou
   if (p->N_ah > 3 || p->N_ah < 1 || k < 0 || k >= p->N_ah) return OA_ERR_ARG;   /* :48-57 */
   if (p->eqsym && p->N_ah == 3) return OA_ERR_PARAM;                              /* :62-71 */
   if (p->z_ah1 < 0.0 || p->z_ah2 > 0.0) return OA_ERR_PARAM;                      /* :76-95 */
-  if (p->ahmove || p->ahmethod != OA_AH_SHOOT) return OA_ERR_UNSUPPORTED;
+  if (p->ahmethod != OA_AH_SHOOT) return OA_ERR_UNSUPPORTED;
   double zah[3] = {0.0, 0.0, 0.0};                                                 /* :104-111 */
   if (p->N_ah > 1) {
     zah[1] = p->z_ah1;
     zah[2] = p->z_ah2;
+  }
+  /* moving centres (:117-180): from z_ah(k), step along the axis by dz0
+   * towards larger phi until Dz_phi changes sign, then place the centre at
+   * the linear zero of Dz_phi.  The level is the finest one holding the
+   * starting point.  A walk that leaves that level (the reference then stops
+   * in interp) returns OA_ERR_STATE. */
+  if (p->ahmove && p->N_ah > 1 && k > 0) {
+    const double r0 = 0.0;
+    double z0 = zah[k], z0_old = 0.0;
+    const int level = ah_level(H, r0, z0);
+    const or_level* L = &H->L[level];
+    double Dzphi = 0.0, Dzphi_old = 0.0;
+    int bracket = 0;
+    for (int it = 0; !bracket; ++it) {
+      if (it > 1000000 || z0 < L->zmin || z0 > L->zmax) return OA_ERR_STATE;
+      Dzphi = or_interp(p, L, L->Dz_phi, r0, z0);
+      if (Dzphi * Dzphi_old >= 0.0) {
+        z0_old = z0;
+        Dzphi_old = Dzphi;
+        z0 = (Dzphi > 0.0) ? z0 + p->dz0 : z0 - p->dz0;
+      } else {
+        bracket = 1;
+        zah[k] = z0_old - Dzphi_old * (z0 - z0_old) / (Dzphi - Dzphi_old);
+      }
+    }
   }
   ah_curv* K = ah_curv_new(H);
   ah_shoot(H, K, zah[k], found, out, theta_out, rad_out);

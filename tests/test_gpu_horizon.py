@@ -300,3 +300,24 @@ def test_critical_collapse_bisection(oaxis_mod):
     assert r2["amplitudes"][0] == r1["amplitudes"][k - 1] and r2["amplitudes"][-1] == r1["amplitudes"][k]
     assert abs((res["hi"] - res["lo"]) - 0.6 / 36) < 1e-12
     assert res["lo"] < 0.6 and res["hi"] > 0.3
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_moving_centres_match_oracle(oaxis_mod, oracle_mod, k):
+    """ahmove on Brill-Lindquist punctures at z = +-3: the centre walk (Dz_phi
+    interpolated on the device) and the search agree with the oracle."""
+    from test_oracle_horizon import BASE, BINARY
+    h = 0.05
+    n = int(round(4.8 / h))
+    p = oaxis_mod.params_from_text(BASE % (".false.", h, h, n, 2 * n) + BINARY)
+    orc = oracle_mod.Oracle(p)
+    orc.initial_data()
+    st = device_from_oracle(oaxis_mod, orc, p)
+    orc.initial_aux()
+    st.initial_aux()
+    dev = st.ahfind(k, shape=True)
+    ref = orc.ahfind(k, shape=True)
+    st.close()
+    orc.close()
+    assert dev["found"] == 1 and abs(abs(dev["zcenter"]) - 3.0) < h
+    compare(dev, ref, "ahmove k=%d" % k)

@@ -198,3 +198,37 @@ def test_off_centre_horizons(oaxis_mod, oracle_mod, k, z0):
     em = [abs(r["mass"] - M) for r in runs]
     assert max(em) < 1e-4 and ez[1] < 1e-6
     assert_ratio(ez[0], ez[1], "zinit")
+
+
+BINARY = """
+idata = BrillLindquist
+N_BH = 2
+BH1mass = 1.0
+BH1Z = 3.0
+BH2mass = 1.0
+BH2Z = -3.0
+N_ah = 3
+z_ah1 = 2.62
+z_ah2 = -2.87
+ahmove = .true.
+"""
+
+
+def test_moving_centres_find_the_punctures(oaxis_mod, oracle_mod):
+    """Brill-Lindquist punctures of mass 1 at z = +-3: with ahmove the centres
+    of horizons 1 and 2 walk from z_ah1, z_ah2 to the maximum of phi on the
+    axis (the puncture, within one coarse cell), and each individual horizon
+    is found around it with a mass near 1."""
+    h = 0.05
+    n = int(round(4.8 / h))
+    p = oaxis_mod.params_from_text(BASE % (".false.", h, h, n, 2 * n) + BINARY)
+    o = oracle_mod.Oracle(p)
+    o.initial_data()
+    o.run_auxiliary(0)
+    for k, zp in ((1, 3.0), (2, -3.0)):
+        res = o.ahfind(k, shape=True)
+        assert abs(res["zcenter"] - zp) < h, (k, res["zcenter"])
+        assert res["found"] == 1 and 0.9 < res["mass"] < 1.3, (k, res["mass"])
+        # the horizon encloses its puncture and stays on its own side
+        assert res["rad"].max() < abs(zp)
+    o.close()
