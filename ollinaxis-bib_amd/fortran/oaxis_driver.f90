@@ -2,7 +2,9 @@
 ! the reference do for the evolution, with the substep body replaced by the
 ! C ABI.  Usage:  oaxis_driver <parfile> <coarse steps> <output file>
 ! Writes alpha, chi and trK of level 0 as raw doubles in the reference
-! layout (1-ghost:Nr, 1-ghost:Nz), column-major, and prints a summary line.
+! layout (1-ghost:Nr, 1-ghost:Nz), column-major, prints a summary line, then
+! looks for the apparent horizon at the final time (horizon_finder, as
+! evolve.f90:429-432 calls it) and prints what ahshoot reports.
 program oaxis_driver
   use, intrinsic :: iso_c_binding
   use oaxis_mod
@@ -15,6 +17,7 @@ program oaxis_driver
   integer(c_int) :: s
   real(c_double), allocatable, target :: alpha(:,:), chi(:,:), trK(:,:)
   character(kind=c_char) :: msg(512)
+  type(oa_horizon) :: ah
   integer :: u
 
   call get_command_argument(1, parfile)
@@ -47,5 +50,8 @@ program oaxis_driver
   close(u)
   print '(a,i0,a,i0,a,i0,a,i0,a,es24.16,a,es24.16,a,f10.3)', 'levels ', nlev, ' Nr ', Nr, ' Nz ', Nz, &
       ' steps ', s, ' t ', t, ' min(alpha) ', minval(alpha), ' ms ', ms
+  call oaxis_check(oa_ahfind(h, 0_c_int, ah), "oa_ahfind")
+  print '(a,i0,a,i0,a,es24.16,a,es24.16,a,es24.16)', 'horizon found ', ah%found, ' shots ', ah%shots, &
+      ' zinit ', ah%zinit, ' area ', ah%area, ' mass ', ah%mass
   call oaxis_check(oa_destroy(h), "oa_destroy")
 end program oaxis_driver

@@ -55,6 +55,12 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path, oaxis_mod, oracle_mo
         st.evolve(STEPS)
         shape = st.shape(0)
         want = {n: st.download(n, 0) for n in ("alpha", "chi", "trK")}
+        ah = st.ahfind(0)
+    # the horizon line: the driver's oa_ahfind on the same state
+    line = [l for l in res.stdout.splitlines() if l.startswith("horizon found")][0].split()
+    assert int(line[2]) == ah["found"] == 1 and int(line[4]) == ah["shots"]
+    for key, pos in (("zinit", 6), ("area", 8), ("mass", 10)):
+        assert float(line[pos]) == ah[key], (key, line[pos], ah[key])
     got = np.fromfile(out, dtype=np.float64).reshape(3, shape[1], shape[0])
     for k, n in enumerate(("alpha", "chi", "trK")):
         assert np.array_equal(got[k].T, want[n]), n
