@@ -513,7 +513,7 @@ void or_sources_geometry(const oa_params* p, or_level* L) {
   const int shift = has_shift(p);
   const int matter = (p->mattertype != 0);
   const double sigma = (p->bssnflavor == OA_LAGRANGIAN) ? 1.0 : 0.0;
-  const double half = 0.5, third = 1.0 / 3.0, sixth = 1.0 / 6.0, two = 2.0, six = 6.0;
+  const double third = 1.0 / 3.0, sixth = 1.0 / 6.0, two = 2.0, six = 6.0;
   const double smallpi = acos(-1.0);
   const double cp = (double)p->chipower;
 
