@@ -56,20 +56,52 @@ __device__ __forceinline__ bool interior_tile(const KGrid& G, int i0, int j0) {
   return i0 >= 1 && i0 + kTile - 1 <= G.Nr - 3 && j0 >= 1 && j0 + kTile - 1 <= G.Nz - 3;
 }
 
-// Stage one RHS input tile: nst planes, plane f from src(f), into LDS.
-template <class Src>
-__device__ __forceinline__ void stage_tile(const KGrid& G, double* tile, int nst, Src src, int ib, int jb) {
+// Staging an RHS input tile into LDS.  Each thread owns the same two points
+// of every staged plane, q0 = tid and q1 = tid + 256 (the tile has 484), so
+// their offsets in a plane are worked out once.  Points of the tile outside
+// the level, which no stencil reads, take the value of the nearest point
+// inside it, so that every load is in bounds and none needs a branch.
+struct StageSlots {
+  long o0, o1;          // plane offsets of the two points
+  int q0, q1;           // their tile positions
+  bool two;             // q1 lies inside the tile
+};
+
+__device__ __forceinline__ StageSlots stage_slots(const KGrid& G, int ib, int jb) {
+  StageSlots s;
   const int tid = static_cast<int>(threadIdx.y) * kTile + static_cast<int>(threadIdx.x);
   const int lo = 1 - G.g;
-  for (int f = 0; f < nst; ++f) {
+  s.q0 = tid;
+  s.q1 = tid + kTile * kTile;
+  s.two = s.q1 < kTileSize;
+  auto offset = [&](int q) {
+    const int ii = min(max(ib + q / kTileW, lo), G.Nr), jj = min(max(jb + q % kTileW, lo), G.Nz);
+    return IDX(G, ii, jj);
+  };
+  s.o0 = offset(s.q0);
+  s.o1 = s.two ? offset(s.q1) : s.o0;
+  return s;
+}
+
+// Stage NST planes, plane f from src(f), into tile slots first .. first +
+// NST - 1.  Every load of the batch is issued before the first LDS store, so
+// their latencies overlap instead of adding up (a load-wait-store loop per
+// plane waits out one memory round trip per plane and point).
+template <int NST, class Src>
+__device__ __forceinline__ void stage_planes(double* tile, int first, const StageSlots& s, Src src) {
+  double v0[NST], v1[NST];
+#pragma unroll
+  for (int f = 0; f < NST; ++f) {
     const double* __restrict__ p = src(f);
-    double* dst = tile + f * kTileSize;
-    for (int q = tid; q < kTileSize; q += kTile * kTile) {
-      const int ii = ib + q / kTileW, jj = jb + q % kTileW;
-      if (ii >= lo && ii <= G.Nr && jj >= lo && jj <= G.Nz) dst[q] = p[IDX(G, ii, jj)];
-    }
+    v0[f] = p[s.o0];
+    v1[f] = p[s.o1];
   }
-  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < NST; ++f) {
+    double* dst = tile + (first + f) * kTileSize;
+    dst[s.q0] = v0[f];
+    if (s.two) dst[s.q1] = v1[f];
+  }
 }
 
 // Pointers read from a work item are generic; every one of them addresses
@@ -126,11 +158,14 @@ __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, doub
   const int i0 = tile_start(bi, nbi, 1, G.Nr);
   const int ib = i0 - kHalo, jb = j0 - kHalo;
   // the evolved fields, then (first substep only) the dtbeta planes it differentiates
-  const int nst = nf + (S.dtb_pre ? 2 : 0);
   const double* __restrict__ yin = gmem(it.yin);
   double* const gaux = gmem(it.gaux);
-  stage_tile(G, tile, nst, [&](int f) { return (f < nf) ? yin + f * G.plane : S.dtb_pre + (f - nf) * G.plane; },
-             ib, jb);
+  {
+    const StageSlots ss = stage_slots(G, ib, jb);
+    stage_planes<nf>(tile, 0, ss, [&](int f) { return yin + f * G.plane; });
+    if (S.dtb_pre) stage_planes<2>(tile, nf, ss, [&](int f) { return S.dtb_pre + f * G.plane; });
+    __syncthreads();
+  }
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
   if (bi == nbi - 1 && i < 1 + (nbi - 1) * kTile) return;        // done by the tile before
   if (bj == nbj - 1 && j < G.jlo + (nbj - 1) * kTile) return;
@@ -195,7 +230,8 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_aux(RhsItem it) {
   const int i0 = tile_start(bi, it.nbi, 1, G.Nr);
   const int ib = i0 - kHalo, jb = j0 - kHalo;
   const double* __restrict__ yin = it.yin;
-  stage_tile(G, tile, NBASE, [&](int f) { return yin + f * G.plane; }, ib, jb);
+  stage_planes<NBASE>(tile, 0, stage_slots(G, ib, jb), [&](int f) { return yin + f * G.plane; });
+  __syncthreads();
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
   if (bi == it.nbi - 1 && i < 1 + (it.nbi - 1) * kTile) return;     // done by the tile before
   if (bj == it.nbj - 1 && j < G.jlo + (it.nbj - 1) * kTile) return;
