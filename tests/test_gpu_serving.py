@@ -93,3 +93,59 @@ def test_serving_results_equal_alone(oaxis_mod):
         a, b = got[r.rid], alone[r.rid]
         bad = [k for k in a if not np.array_equal(a[k], b[k])]
         assert not bad, "request %d (%s) differs in the mix: %s" % (r.rid, r.kind, bad[:6])
+
+
+DISPERSING_PULSE = """
+order = four
+dtfac = 0.25
+boundtype = radiative
+integrator = rk4
+eqsym = .false.
+dr0 = 0.25
+dz0 = 0.25
+Nrtotal = 32
+Nztotal = 64
+Nl0 = 0
+mattertype = scalar
+idata = scalarpulse
+scalarpulse_type = shell
+scalar_a0 = 0.001
+scalar_r0 = 0.0
+scalar_z0 = 0.0
+scalar_sr0 = 1.0
+scalar_sz0 = 1.0
+"""
+
+
+def test_serving_retires_a_dispersed_pulse(oaxis_mod):
+    """A weak central scalar pulse spreads out and leaves through the
+    radiative boundary: the request retires as "dispersed" at the first
+    check (every check_every coarse steps) where max |scalar_phi| on level 0
+    is below disperse_frac of its initial value (the oracle's evolution of
+    this pulse crosses 0.1 at step 48, after a dip to 0.17 and a rebound to
+    0.45).  The same pulse with a smaller step count retires at its final
+    time first."""
+    oaxis = oaxis_mod
+    from oaxis.serve import Request, Server
+    frac, every, cap = 0.1, 4, 400
+    p = oaxis.params_from_text(DISPERSING_PULSE)
+    # the step of the first check that sees the field dispersed, evolving alone
+    with oaxis.Spacetime(p) as st:
+        st.initial_data()
+        st.initial_aux()
+        phi0 = st.norms0d("scalar_phi", 0)["nm1"]
+        expect = None
+        for k in range(every, cap, every):
+            st.evolve(every)
+            if st.norms0d("scalar_phi", 0)["nm1"] < frac * phi0:
+                expect = k
+                break
+    print("dispersing pulse: phi0 %.3e, dispersed at step %s" % (phi0, expect), flush=True)
+    assert phi0 > 0.0 and expect == 48
+    reqs = [Request(0, DISPERSING_PULSE, cap, 0.0, "scalar_pulse"),
+            Request(1, DISPERSING_PULSE, expect - every // 2, 0.0, "scalar_pulse")]
+    srv = Server(1e10, check_every=every, disperse_frac=frac)
+    records, summary = srv.run(reqs)
+    assert summary["requests"] == 2
+    assert records[0].reason == "dispersed" and records[0].steps == expect, (records[0].reason, records[0].steps)
+    assert records[1].reason == "final_time" and records[1].steps == expect - every // 2
