@@ -123,6 +123,9 @@ struct FbItem {                   // finebound of a fine level
 };
 struct RsItem {                   // restriction of a fine level onto its parent
   KGrid Gf, Gc; int nf, interporder; const double* yf; double* yc;
+  // coarse points ic0 .. ic0 + nic - 1, jc0 .. jc0 + njc - 1 (inside the
+  // coarse level): a box around the points the fine level restricts onto
+  int ic0, nic, jc0, njc;
   int block0;
 };
 

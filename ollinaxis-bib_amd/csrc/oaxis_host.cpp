@@ -3,6 +3,7 @@
 // in the reference layout, and the C ABI declared in include/oaxis.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -270,6 +271,29 @@ void prepare_restrict(oa_handle* h, int level, RsItem* it) {
   it->interporder = h->p.interporder;
   it->yf = L.slab[L.cur];
   it->yc = Lc.slab[Lc.cur];
+  // The coarse points the kernel may restrict onto: r0 in (rmin, rmax), z0 in
+  // (zmin, zmax), with the bounds of restrict_point (restrict.f90:88-162).
+  // The box adds a point of margin on each side and is clipped to the coarse
+  // level; the kernel still tests every point against the exact bounds.
+  const KGrid& Gf = it->Gf;
+  const KGrid& Gc = it->Gc;
+  const int g = Gc.g;
+  const double deltar = 2.0 * g * Gf.dr + 1.0e-10, deltaz = 2.0 * g * Gf.dz + 1.0e-10;
+  const double rminf = (1 - g - 0.5) * Gf.dr, rmaxf = (Gf.Nr - 0.5) * Gf.dr;
+  const double zminf = (1 - g - Gf.zoff) * Gf.dz, zmaxf = (Gf.Nz - Gf.zoff) * Gf.dz;
+  const double rmin = (rminf < 0.0) ? 0.0 : rminf + deltar, rmax = rmaxf - deltar;
+  const double zmin = (Gf.eqsym && zminf < 0.0) ? 0.0 : zminf + deltaz, zmax = zmaxf - deltaz;
+  // coarse r_i = (i - 1/2) dr, z_j = (j - zoff) dz
+  const int ilo = std::max(1 - g, static_cast<int>(std::floor(rmin / Gc.dr + 0.5)) - 1);
+  const int ihi = std::min(Gc.Nr, static_cast<int>(std::ceil(rmax / Gc.dr + 0.5)) + 1);
+  const int jlo = std::max(1 - g, static_cast<int>(std::floor(zmin / Gc.dz + Gc.zoff)) - 1);
+  const int jhi = std::min(Gc.Nz, static_cast<int>(std::ceil(zmax / Gc.dz + Gc.zoff)) + 1);
+  it->ic0 = ilo;
+  it->nic = std::max(1, ihi - ilo + 1);
+  it->jc0 = jlo;
+  it->njc = std::max(1, jhi - jlo + 1);
+  if (it->ic0 + it->nic - 1 > Gc.Nr) it->nic = Gc.Nr - it->ic0 + 1;
+  if (it->jc0 + it->njc - 1 > Gc.Nz) it->njc = Gc.Nz - it->jc0 + 1;
   Lc.dirty = true;
   Lc.cons_valid = false;
 }

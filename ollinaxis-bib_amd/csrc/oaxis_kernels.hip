@@ -332,16 +332,16 @@ __device__ __forceinline__ void finebound_body(const FbItem& it, int blk) {
 }
 
 // restrict: blocks of one field, then the next; one thread per coarse point
+// of the item's box (the points of the coarse level the fine level may
+// restrict onto; restrict_point tests each against the exact bounds)
 __device__ __forceinline__ void restrict_body(const RsItem& it, int blk) {
-  const KGrid& Gc = it.Gc;
-  const int nzt = Gc.Nz + Gc.g;
-  const long per = static_cast<long>(Gc.Nr + Gc.g) * nzt;
+  const long per = static_cast<long>(it.nic) * it.njc;
   const int bpf = static_cast<int>((per + kPtBlock - 1) / kPtBlock);
   const int s = blk / bpf;
   const long q = static_cast<long>(blk % bpf) * kPtBlock + threadIdx.x;
   if (s >= it.nf || q >= per) return;
-  const int ic = static_cast<int>(q / nzt) + 1 - Gc.g, jc = static_cast<int>(q % nzt) + 1 - Gc.g;
-  restrict_point(it.Gf, Gc, s, it.interporder, gmem(it.yf), gmem(it.yc), ic, jc);
+  const int ic = it.ic0 + static_cast<int>(q / it.njc), jc = it.jc0 + static_cast<int>(q % it.njc);
+  restrict_point(it.Gf, it.Gc, s, it.interporder, gmem(it.yf), gmem(it.yc), ic, jc);
 }
 
 template <bool SHIFT, bool SCALAR>
@@ -468,9 +468,7 @@ int save_blocks(const SaveItem& it) {
 int fb_blocks(const FbItem& it, bool shift, bool scalar) {
   return finebound_groups(shift, scalar) * blocks_for(fb_count(it.Gf, it.Gf.g + 2));
 }
-int rs_blocks(const RsItem& it) {
-  return it.nf * blocks_for(static_cast<long>(it.Gc.Nr + it.Gc.g) * (it.Gc.Nz + it.Gc.g));
-}
+int rs_blocks(const RsItem& it) { return it.nf * blocks_for(static_cast<long>(it.nic) * it.njc); }
 
 template <class K>
 static int raise_lds_limit(K* kernel, size_t bytes) {
