@@ -192,7 +192,7 @@ def run_serving(args, n, steps_scale, max_grid):
     grids = tuple(g for g in ((64, 128), (128, 256), (256, 512), (512, 1024)) if g[0] <= max_grid)
     lo, hi = max(1, int(200 * steps_scale)), max(1, int(2000 * steps_scale))
     reqs = serve.serving_mix(args.seed, n, rate=args.serving_rate, grids=grids, steps=(lo, hi))
-    srv = serve.Server(memory_budget=args.serving_budget_gb * 1e9)
+    srv = serve.Server(memory_budget=args.serving_budget_gb * 1e9, graphs=not args.no_graphs)
     records, summary = srv.run(reqs)
     summary["sample"] = ("%d requests of the serving mix (seed %d, Poisson rate %.3g/s, grids up to %dx%d, "
                          "1-6 levels, %d-%d coarse steps, horizon search every 16), memory budget %.0f GB"

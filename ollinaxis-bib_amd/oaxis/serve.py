@@ -59,7 +59,7 @@ class Record:
 
 class Server:
     def __init__(self, memory_budget, max_members=256, check_every=16, alpha_collapse=0.05,
-                 disperse_frac=0.01, on_retire=None):
+                 disperse_frac=0.01, on_retire=None, graphs=True):
         import oaxis
         self.oaxis = oaxis
         self.budget = float(memory_budget)
@@ -68,6 +68,7 @@ class Server:
         self.alpha_collapse = alpha_collapse
         self.disperse_frac = disperse_frac
         self.on_retire = on_retire       # callback(request, spacetime, record) before the state is freed
+        self.graphs = bool(graphs)       # coarse steps as captured graphs (False: kernel by kernel)
 
     def _admit(self, req, now, batch, active, used):
         oaxis = self.oaxis
@@ -155,8 +156,9 @@ class Server:
         used = 0.0
         batch = oaxis.Batch()
         # the membership changes too often for the schedule autotuner to
-        # settle: a fixed schedule (graphs; RHS blocks renumbered per XCD)
-        batch.set_graphs(True)
+        # settle: a fixed schedule (graphs unless switched off; RHS blocks
+        # renumbered per XCD)
+        batch.set_graphs(self.graphs)
         t0 = clock()
         total_updates = 0.0
         try:
