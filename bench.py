@@ -120,9 +120,19 @@ def cpu_baseline(texts, min_seconds, threads=1):
     tally = [(0.0, 0)] * len(owned)
     start = threading.Barrier(len(owned) + 1)
 
+    errors = []
+
     def worker(t):
-        orcs[t] = [make(text) for text in owned[t]]
-        start.wait()
+        try:
+            orcs[t] = [make(text) for text in owned[t]]
+        except Exception as e:          # let the other threads and the caller go on
+            errors.append(e)
+            start.abort()
+            return
+        try:
+            start.wait()
+        except threading.BrokenBarrierError:
+            return
         t0 = time.perf_counter()
         updates, steps, k = 0.0, 0, 0
         while time.perf_counter() - t0 < min_seconds:
@@ -136,7 +146,10 @@ def cpu_baseline(texts, min_seconds, threads=1):
     pool = [threading.Thread(target=worker, args=(t,)) for t in range(len(owned))]
     for th in pool:
         th.start()
-    start.wait()
+    try:
+        start.wait()
+    except threading.BrokenBarrierError:
+        pass
     t0 = time.perf_counter()
     for th in pool:
         th.join()
@@ -144,6 +157,8 @@ def cpu_baseline(texts, min_seconds, threads=1):
     for mine in orcs:
         for orc, _ in mine or []:
             orc.close()
+    if errors:
+        raise errors[0]
     return sum(u for u, _ in tally) / dt, dt, sum(n for _, n in tally)
 
 

@@ -20,3 +20,15 @@ def test_cpu_baseline_one_and_several_threads(oracle_mod):
     rec = bench.cpu_baseline_record(texts, 0.2, "two Schwarzschild instances")
     assert rec["kind"] == "port" and rec["cores"] == bench.host_threads()
     assert rec["value"] > 0.0 and rec["one_core"]["value"] > 0.0
+
+
+def test_cpu_baseline_reports_a_setup_error_instead_of_waiting(oracle_mod):
+    """A worker thread whose instance cannot be built releases the others
+    (they start together) and the error reaches the caller."""
+    import pytest
+    import bench
+    import oaxis
+    from oaxis import workloads
+    texts = [workloads.schwarzschild(1.0, 32, 64, 0), "no_such_key = 1\n"]
+    with pytest.raises(oaxis.OaxisError):
+        bench.cpu_baseline(texts, 0.2, 2)
