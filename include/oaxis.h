@@ -259,6 +259,21 @@ int oa_constraints(oa_handle* h, int box, int level);
  * reference's serial loop, so nm2 agrees to rounding).  A NaN in the range
  * returns OA_ERR_STATE (the reference stops the run there). */
 int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc): the 0D norms of n arrays (1 <= n <=
+ * OA_NORMS_NOALLOC_MAX) of one level, each formed as oa_norms0d forms it,
+ * written to DEVICE memory out_dev: 5 doubles per array, in the order of
+ * names: max, min, nm1, nm2, and 1.0 if the range holds a NaN (else 0.0).
+ * Nothing is allocated: the per-block partial sums live in one fixed device
+ * array of the library per GPU (OA_NORMS_NOALLOC_MAX x 256 records of 6
+ * doubles, 384 KiB, held until the process ends), every record of which a
+ * call rewrites before reading it.  The call returns once the work is
+ * enqueued on the handle's stream (results are ready after oa_sync).  Calls
+ * that overlap on one GPU (other handles' streams, other threads) share that
+ * array and race. */
+#define OA_NORMS_NOALLOC_MAX 32
+int oa_norms0d_noalloc(oa_handle* h, int box, int level, int n, const char* const* names, double* out_dev);
 /* Checkpoint and restart (checkpointsave / checkpointload,
  * src/base/checkpoint.f90:2-690).  The reference writes every evolved array,
  * the fine-boundary time levels var_bound_* with the clocks t, t1, t2 of each

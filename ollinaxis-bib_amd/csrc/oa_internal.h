@@ -162,6 +162,12 @@ int launch_constraints(const KGrid& G, const KPhys& P, const KStep& S, bool phim
 // 0D norms (save0Ddata): norms_blocks() partial records of 6 doubles
 int norms_blocks();
 int launch_norms_partial(const KGrid& G, const double* f, double* part, void* stream);
+// the same norms for n fields of one level, with the partial records in the
+// library's fixed device array on device dev (no allocation); out is device
+// memory of 5 doubles per field (max, min, max|f|, rms, NaN flag)
+int norms_max_fields();
+int launch_norms_fields_noalloc(int dev, const KGrid& G, int n, const double* const* planes, double* out,
+                                void* stream);
 int launch_dtbeta_init(const KGrid& G, double csi, bool scalar, double* y, void* stream);
 int launch_fb_table(const KGrid& Gf, const KGrid& Gc, int depth, int interporder, FbPoint* table, long n,
                     void* stream);

@@ -48,6 +48,7 @@ struct oa_handle {
   bool shift = false, scalar = false;
   double dt0 = 0.0;
   std::vector<oax::Level> L;
+  int device = 0;                    // the HIP device holding the state
   hipStream_t stream = nullptr;      // the stream every call of this handle uses
   hipStream_t own_stream = nullptr;  // its own; a batch lends its stream while the handle is a member
   oa_batch* batch = nullptr;

@@ -410,6 +410,11 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
     oa_set_error("no HIP device available: this library runs on the GPU only");
     return OA_ERR_HIP;
   }
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    oa_set_error("hipGetDevice failed");
+    return OA_ERR_HIP;
+  }
   if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev_begin) != hipSuccess || hipEventCreate(&h->ev_end) != hipSuccess) {
     delete h;
@@ -639,21 +644,31 @@ extern "C" int oa_constraints(oa_handle* h, int box, int level) {
   return OA_OK;
 }
 
+namespace {
+// the plane oa_norms0d reduces for `name`: an evolved array whose stored
+// values are current, or a constraint computed on the current state
+int norms_plane(oa_handle* h, Level* L, const char* caller, const char* name, const double** out) {
+  const int slot = slot_of(h, name);
+  const int cslot = cons_slot(name);
+  if (slot >= 0 && !(slot == S_phi && !L->phi_valid)) {
+    *out = L->slab[L->in_step ? L->last_out : L->cur] + static_cast<long>(slot) * L->G.plane;
+    return OA_OK;
+  }
+  if (cslot >= 0 && L->cons_valid) {
+    *out = L->cons + static_cast<long>(cslot) * L->G.plane;
+    return OA_OK;
+  }
+  oa_set_error(std::string(caller) + ": " + name +
+               " is not a stored evolved array of this configuration (constraints need oa_constraints first)");
+  return OA_ERR_ARG;
+}
+}  // namespace
+
 extern "C" int oa_norms0d(oa_handle* h, int box, int level, const char* name, double* out) {
   Level* L = level_of(h, box, level);
   if (!L || !name || !out) return OA_ERR_ARG;
-  const int slot = slot_of(h, name);
-  const int cslot = cons_slot(name);
   const double* plane = nullptr;
-  if (slot >= 0 && !(slot == S_phi && !L->phi_valid)) {
-    plane = L->slab[L->in_step ? L->last_out : L->cur] + static_cast<long>(slot) * L->G.plane;
-  } else if (cslot >= 0 && L->cons_valid) {
-    plane = L->cons + static_cast<long>(cslot) * L->G.plane;
-  } else {
-    oa_set_error(std::string("oa_norms0d: ") + name +
-                 " is not a stored evolved array of this configuration (constraints need oa_constraints first)");
-    return OA_ERR_ARG;
-  }
+  RC(norms_plane(h, L, "oa_norms0d", name, &plane));
   RC(launch_norms_partial(L->G, plane, h->norms_part, h->stream));
   const int nb = norms_blocks();
   std::vector<double> part(static_cast<size_t>(6 * nb));
@@ -678,6 +693,29 @@ extern "C" int oa_norms0d(oa_handle* h, int box, int level, const char* name, do
   out[2] = vabs;
   out[3] = (cnt > 0.0) ? std::sqrt(sq / cnt) : 0.0;
   return OA_OK;
+}
+
+extern "C" int oa_norms0d_noalloc(oa_handle* h, int box, int level, int n, const char* const* names,
+                                  double* out_dev) {
+  Level* L = level_of(h, box, level);
+  if (!L || !names || !out_dev) {
+    if (L) oa_set_error("oa_norms0d_noalloc: null argument");
+    return OA_ERR_ARG;
+  }
+  if (n < 1 || n > norms_max_fields()) {
+    oa_set_error("oa_norms0d_noalloc: " + std::to_string(n) + " arrays requested; one call takes 1 to " +
+                 std::to_string(norms_max_fields()));
+    return OA_ERR_ARG;
+  }
+  const double* planes[64];
+  for (int k = 0; k < n; ++k) {
+    if (!names[k]) {
+      oa_set_error("oa_norms0d_noalloc: null array name");
+      return OA_ERR_ARG;
+    }
+    RC(norms_plane(h, L, "oa_norms0d_noalloc", names[k], &planes[k]));
+  }
+  return launch_norms_fields_noalloc(h->device, L->G, n, planes, out_dev, h->stream);
 }
 
 extern "C" int oa_params_footprint(const oa_params* pin, double* bytes) {

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "oa_internal.h"
 
@@ -376,14 +378,14 @@ __global__ void __launch_bounds__(kPtBlock) k_restrict_table(const RsItem* __res
 // point count and a NaN flag, in a fixed tree order; the host combines the
 // blocks in block order.
 constexpr int kNormBlocks = 256;
-__global__ void __launch_bounds__(kPtBlock) k_norms_partial(KGrid G, const double* __restrict__ f,
-                                                            double* __restrict__ part) {
+// partial record b of one field: a grid-stride pass of block b of nb
+__device__ __forceinline__ void norms_block(const KGrid& G, const double* __restrict__ f, double* __restrict__ part,
+                                            int b, int nb) {
   __shared__ double smax[kPtBlock], smin[kPtBlock], sabs[kPtBlock], ssq[kPtBlock], scnt[kPtBlock], snan[kPtBlock];
   const int ni = G.Nr - G.g, nj = G.Nz - G.g;
   const long n = static_cast<long>(ni) * nj;
   double vmax = -1.0e300, vmin = 1.0e300, vabs = 0.0, sq = 0.0, cnt = 0.0, nan = 0.0;
-  for (long q = static_cast<long>(blockIdx.x) * kPtBlock + threadIdx.x; q < n;
-       q += static_cast<long>(gridDim.x) * kPtBlock) {
+  for (long q = static_cast<long>(b) * kPtBlock + threadIdx.x; q < n; q += static_cast<long>(nb) * kPtBlock) {
     const int i = 1 + static_cast<int>(q / nj), j = 1 + static_cast<int>(q % nj);
     const double v = f[IDX(G, i, j)];
     if (v != v) nan = 1.0;
@@ -408,9 +410,54 @@ __global__ void __launch_bounds__(kPtBlock) k_norms_partial(KGrid G, const doubl
     __syncthreads();
   }
   if (t == 0) {
-    double* o = part + 6 * blockIdx.x;
+    double* o = part + 6 * b;
     o[0] = smax[0]; o[1] = smin[0]; o[2] = sabs[0]; o[3] = ssq[0]; o[4] = scnt[0]; o[5] = snan[0];
   }
+}
+
+__global__ void __launch_bounds__(kPtBlock) k_norms_partial(KGrid G, const double* __restrict__ f,
+                                                            double* __restrict__ part) {
+  norms_block(G, f, part, static_cast<int>(blockIdx.x), kNormBlocks);
+}
+
+// The norms of several fields of one level: blockIdx.y picks the field,
+// whose kNormBlocks partial records start at part + y * 6 * kNormBlocks.
+constexpr int kNormsMaxFields = 32;
+struct NormsFields {
+  KGrid G;
+  int n;
+  const double* f[kNormsMaxFields];
+};
+
+__global__ void __launch_bounds__(kPtBlock) k_norms_fields(NormsFields a, double* __restrict__ part) {
+  const int y = static_cast<int>(blockIdx.y);
+  norms_block(a.G, a.f[y], part + static_cast<long>(6) * kNormBlocks * y, static_cast<int>(blockIdx.x),
+              kNormBlocks);
+}
+
+// Thread y combines the partial records of field y in block order, as the
+// host does for oa_norms0d: out[5y..5y+4] = max, min, max|f|, rms, NaN flag.
+__global__ void __launch_bounds__(64) k_norms_combine(int n, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  const int y = static_cast<int>(threadIdx.x);
+  if (y >= n) return;
+  const double* p = part + static_cast<long>(6) * kNormBlocks * y;
+  double vmax = -1.0e300, vmin = 1.0e300, vabs = 0.0, sq = 0.0, cnt = 0.0, nan = 0.0;
+  for (int b = 0; b < kNormBlocks; ++b) {
+    const double* o = p + 6 * b;
+    vmax = fmax(vmax, o[0]);
+    vmin = fmin(vmin, o[1]);
+    vabs = fmax(vabs, o[2]);
+    sq += o[3];
+    cnt += o[4];
+    nan = fmax(nan, o[5]);
+  }
+  double* r = out + 5 * y;
+  r[0] = vmax;
+  r[1] = vmin;
+  r[2] = vabs;
+  r[3] = (cnt > 0.0) ? sqrt(sq / cnt) : 0.0;
+  r[4] = nan;
 }
 
 // dtbeta = drivercsi * Delta (evolve.f90:174-195), whole arrays incl. images
@@ -589,6 +636,71 @@ int norms_blocks() { return kNormBlocks; }
 int launch_norms_partial(const KGrid& G, const double* f, double* part, void* stream) {
   hipLaunchKernelGGL(k_norms_partial, dim3(kNormBlocks), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), G, f,
                      part);
+  return check_launch();
+}
+
+// Partial records of oa_norms0d_noalloc: kNormsMaxFields fields of
+// kNormBlocks records of 6 doubles (384 KiB).  Each GPU holds its own copy for
+// the life of the process.  Host code never uses this name as an address: it
+// uses the device address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[kNormsMaxFields * kNormBlocks * 6];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the address of `scratch` on device dev, looked up once per device with that
+// device current
+int scratch_on(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    oa_set_error("oa_norms0d_noalloc: device index out of range");
+    return OA_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) {
+      oa_set_error("oa_norms0d_noalloc: hipGetDevice failed");
+      return OA_ERR_HIP;
+    }
+    if (prev != dev && hipSetDevice(dev) != hipSuccess) {
+      oa_set_error("oa_norms0d_noalloc: hipSetDevice failed");
+      return OA_ERR_HIP;
+    }
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != dev) (void)hipSetDevice(prev);
+    if (e != hipSuccess || !ptr) {
+      oa_set_error(std::string("oa_norms0d_noalloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return OA_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return OA_OK;
+}
+}  // namespace
+
+int norms_max_fields() { return kNormsMaxFields; }
+
+int launch_norms_fields_noalloc(int dev, const KGrid& G, int n, const double* const* planes, double* out,
+                                void* stream) {
+  if (n < 1 || n > kNormsMaxFields) {
+    oa_set_error("oa_norms0d_noalloc: " + std::to_string(n) + " arrays requested; one call takes 1 to " +
+                 std::to_string(kNormsMaxFields));
+    return OA_ERR_ARG;
+  }
+  double* part = nullptr;
+  RC_K(scratch_on(dev, &part));
+  NormsFields a{};
+  a.G = G;
+  a.n = n;
+  for (int k = 0; k < n; ++k) a.f[k] = planes[k];
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_norms_fields, dim3(kNormBlocks, static_cast<unsigned>(n)), dim3(kPtBlock), 0, st, a, part);
+  RC_K(check_launch());
+  hipLaunchKernelGGL(k_norms_combine, dim3(1), dim3(64), 0, st, n, part, out);
   return check_launch();
 }
 

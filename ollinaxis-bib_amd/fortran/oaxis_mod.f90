@@ -22,7 +22,7 @@ module oaxis_mod
             oa_saveold, oa_substep, oa_finebound, oa_auxiliary, oa_restrict, oa_advance_time, &
             oa_initial_aux, oa_coarse_step, oa_evolve, oa_sync, oa_initial_data, &
             oa_last_elapsed_ms, oa_point_updates, oa_params_parse_text, oa_set_kernel_timing, &
-            oa_rhs_kernel_stats, oa_norms0d, oa_params_footprint, &
+            oa_rhs_kernel_stats, oa_norms0d, oa_norms0d_noalloc, oa_params_footprint, &
             oa_batch_create, oa_batch_destroy, oa_batch_add, oa_batch_remove, oa_batch_size, &
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
@@ -291,6 +291,15 @@ module oaxis_mod
       integer(c_int), value :: box, level
       character(kind=c_char), dimension(*) :: name
       real(c_double), dimension(4) :: norms
+    end function
+    ! the same norms of n arrays into device memory (5 doubles each), with no
+    ! allocation: names is an array of n pointers to C strings (c_loc of each)
+    integer(c_int) function oa_norms0d_noalloc(h, box, level, n, names, out_dev) bind(C, name="oa_norms0d_noalloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: box, level, n
+      type(c_ptr), dimension(*) :: names
+      type(c_ptr), value :: out_dev
     end function
     integer(c_int) function oa_params_footprint(p, bytes) bind(C, name="oa_params_footprint")
       import :: c_int, c_ptr, c_double

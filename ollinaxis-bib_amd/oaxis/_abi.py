@@ -114,6 +114,8 @@ def load_library(path=LIBPATH):
         "oa_rhs_kernel_stats": (ctypes.c_int, [H, dp, ctypes.POINTER(ctypes.c_long)]),
         "oa_point_updates": (ctypes.c_int, [H, dp]),
         "oa_norms0d": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp]),
+        "oa_norms0d_noalloc": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_char_p), ctypes.c_void_p]),
         "oa_constraints": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
         "oa_ahfind": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(Horizon)]),
         "oa_horizon_shape": (ctypes.c_int, [H, ctypes.c_int, dp, dp, ctypes.c_int, ip]),
