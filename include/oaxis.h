@@ -69,14 +69,15 @@ enum { OA_ISHIFT_ZERO = 0, OA_ISHIFT_SHELL, OA_ISHIFT_TORUS, OA_ISHIFT_QUISO_KER
 enum { OA_POT_NONE = 0, OA_POT_PHI2, OA_POT_PHI4 };
 enum { OA_PULSE_SHELL = 0, OA_PULSE_TORUS, OA_PULSE_BAUMGARTE };
 enum { OA_SCALARMETHOD_FIRST = 1, OA_SCALARMETHOD_SECOND = 2 };
+enum { OA_ELL_NONE = 0, OA_ELL_WAVE, OA_ELL_SOR };
 /* mattertype is a "multiple" key: bit mask */
 enum { OA_MATTER_SCALAR = 1, OA_MATTER_COMPLEX = 2 };
 
 /* Parameters of the evolution hot path.  Member names are the reference's
  * par-file keys and defaults are the reference's (src/base/param.f90:95-800).
  * Keys of the reference that this library does not act on (output, horizon
- * finder tuning, elliptic solver, complex field, ...) are still accepted by the
- * parser so that reference par files load unchanged. */
+ * finder tuning, complex field, ...) are still accepted by the parser so that
+ * reference par files load unchanged. */
 typedef struct oa_params {
   /* grid (param.f90:95-168) */
   int    eqsym;
@@ -130,6 +131,13 @@ typedef struct oa_params {
   int    scalarpotential, scalarpulse_type, scalarmethod;
   double scalar_mass, scalar_lambda;
   double scalar_a0, scalar_r0, scalar_z0, scalar_sr0, scalar_sz0;
+  /* elliptic solve of the Hamiltonian constraint for Brill-wave and
+   * scalar-pulse data (param.f90:454-491) */
+  int    ELL_solver;              /* OA_ELL_*                               */
+  int    ELL_maxiter;
+  double ELL_epsilon;
+  int    WE_method, WE_maxiter;   /* WE_method: OA_ICN or OA_RK4            */
+  double WE_eta, WE_diss, WE_dtfac, WE_epsilon;
 } oa_params;
 
 /* Fill *p with the reference defaults. */
@@ -210,13 +218,27 @@ int oa_evolve(oa_handle* h, int nsteps);
 /* Wait for all device work of the handle. */
 int oa_sync(oa_handle* h);
 
-/* Closed-form initial data on every level (host side, not the hot path):
- * minkowski (+ gauss lapse/shift perturbations), schwarzschild,
- * BrillLindquist, BrillWave (Holz) and scalarpulse, the last two with
- * psi = 1 (no Hamiltonian-constraint solve).  Follows
- * initial.f90:53-150,189-330,528-620, idata_brillwave.f90:88-105,183-208
- * and idata_scalarpulse.f90:60-103. */
+/* Initial data on every level, computed on the device: minkowski (+ gauss
+ * lapse/shift perturbations), schwarzschild, BrillLindquist, and BrillWave
+ * (Holz) and scalarpulse with psi from the WaveElliptic solve of the
+ * Hamiltonian constraint (ELL_solver = wave).  Follows
+ * initial.f90:53-150,189-330,528-620, idata_brillwave.f90:88-208,
+ * idata_scalarpulse.f90:60-206 and wave_elliptic.f90:2-1164. */
 int oa_initial_data(oa_handle* h);
+/* Stopping rule of that solve.  The reference stops when the RMS of dv/dt on
+ * the base level falls below WE_epsilon (after at least 100 steps) or at
+ * WE_maxiter.  With refinement levels that residual stalls at the mismatch
+ * between the coarse and the restricted fine solution, so the reference runs
+ * to WE_maxiter although the solution stops changing after a few thousand
+ * steps.  stationary = 1 (the default; no counterpart in the reference)
+ * also ends the all-levels phase once the largest change of u per step on
+ * the base level, relative to max|u|, has stayed below 1e-12 over 100 steps
+ * and stopped falling; the solution then differs from the one WE_maxiter
+ * steps give by rounding only.  stationary = 0 keeps the reference's rule
+ * alone.  oa_elliptic_stats reports the iterations of the base-level solve
+ * and of the all-levels solve (0 without refinement) and the last residual. */
+int oa_set_elliptic_exit(oa_handle* h, int stationary);
+int oa_elliptic_stats(const oa_handle* h, int* steps0, int* steps1, double* residual);
 
 /* Device time of the last oa_evolve / oa_coarse_step call in ms (HIP events
  * on the handle's stream), and the kernel launch counts. */

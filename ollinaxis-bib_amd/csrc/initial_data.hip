@@ -4,11 +4,13 @@
 // :585-620 (trace removal), :627-745 (precollapsed lapse, gauss lapse and
 // shift perturbations), idata_brillwave.f90:88-105,183-208 (Brill/Holz metric)
 // and idata_scalarpulse.f90:60-103 (scalar pulse).  For Brill waves and the
-// scalar pulse the Hamiltonian constraint is not solved: psi stays 1.
+// scalar pulse psi comes from the WaveElliptic solve of the Hamiltonian
+// constraint (elliptic.hip), which needs the fields of a first pass.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "oa_internal.h"
 #include "oaxis.h"
@@ -35,19 +37,28 @@ __device__ double torus(double a0, double r0, double z0, double sr, double sz, d
 }  // namespace
 
 // the opaque handle is defined in oaxis_host.cpp; initial data only needs
-// these accessors
+// these accessors and the elliptic solve
 namespace oax {
 int handle_level_slab(oa_handle* h, int level, double** slab, KGrid* G, int* nf, bool* shift, bool* scalar,
                       const oa_params** p);
 void handle_mark_dirty(oa_handle* h, int level);
 void* handle_stream(oa_handle* h);
+int handle_elliptic_exit(oa_handle* h);
+void handle_elliptic_record(oa_handle* h, int steps0, int steps1, double residual);
+struct EllSolver;
+int ell_solve_begin(oa_handle* h, int stationary_exit, EllSolver** out);
+int ell_solve_run(EllSolver* s, int* steps0, int* steps1, double* gres);
+const double* ell_solution(const EllSolver* s, int level);
+void ell_solve_end(EllSolver* s);
 }  // namespace oax
 
 namespace {
 
 // One point (i, j) of the level: every slot of the state at that point.
+// psi: the solved conformal factor of Brill-wave / scalar-pulse data on this
+// level (null in the first pass, which only needs the scalar field and xi).
 __global__ void __launch_bounds__(256) k_initial_data(oa_params p, oax::KGrid G, bool shift, bool scalar,
-                                                      double* __restrict__ y) {
+                                                      const double* __restrict__ psi, double* __restrict__ y) {
   using namespace oax;
   const int g = G.g;
   const int j = 1 - g + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -84,8 +95,14 @@ __global__ void __launch_bounds__(256) k_initial_data(oa_params p, oax::KGrid G,
     at(S_phi) = std::log(ps);
     at(S_chi) = 1.0 / std::pow(ps, static_cast<double>(p.chipower));
   }
+  if ((p.idata == OA_IDATA_BRILLWAVE || p.idata == OA_IDATA_SCALARPULSE) && psi) {
+    // idata_brillwave.f90:183-197, idata_scalarpulse.f90:185-206
+    const double sp = psi[IDX(G, i, j)];
+    at(S_phi) = std::log(sp);
+    at(S_chi) = 1.0 / std::pow(sp, static_cast<double>(p.chipower));
+  }
   if (p.idata == OA_IDATA_BRILLWAVE) {
-    // idata_brillwave.f90:88-105,183-208 with psi = 1 (no constraint solve)
+    // idata_brillwave.f90:88-105,198-208
     const double q = p.brill_a0 * r * r *
                      std::exp(-(r / p.brill_sr0) * (r / p.brill_sr0) - (z / p.brill_sz0) * (z / p.brill_sz0));
     at(S_A) = std::exp(2.0 * q);
@@ -179,10 +196,12 @@ __global__ void __launch_bounds__(256) k_initial_xi(oax::KGrid G, int eqsym, int
 
 }  // namespace
 
-extern "C" int oa_initial_data(oa_handle* h) {
+namespace {
+
+// every closed-form field of every level; psi[l] the solved conformal
+// factor of level l, or null
+int fill_levels(oa_handle* h, int nlev, const double* const* psi) {
   using namespace oax;
-  int nlev = 0;
-  if (!h || oa_num_levels(h, &nlev) != OA_OK) return OA_ERR_ARG;
   hipStream_t st = static_cast<hipStream_t>(handle_stream(h));
   for (int l = 0; l < nlev; ++l) {
     double* slab = nullptr;
@@ -200,7 +219,7 @@ extern "C" int oa_initial_data(oa_handle* h) {
     }
     const dim3 blk(64, 4, 1);
     const dim3 grd((G.Nz + g + 63) / 64, (G.Nr + g + 3) / 4, 1);
-    hipLaunchKernelGGL(k_initial_data, grd, blk, 0, st, p, G, shift, scalar, slab);
+    hipLaunchKernelGGL(k_initial_data, grd, blk, 0, st, p, G, shift, scalar, psi ? psi[l] : nullptr, slab);
     if (scalar && p.idata == OA_IDATA_SCALARPULSE)
       hipLaunchKernelGGL(k_initial_xi, grd, blk, 0, st, G, p.eqsym, scalar_slot(shift, SC_phi),
                          scalar_slot(shift, SC_xi_r), scalar_slot(shift, SC_xi_z), slab);
@@ -211,4 +230,40 @@ extern "C" int oa_initial_data(oa_handle* h) {
     handle_mark_dirty(h, l);
   }
   return OA_OK;
+}
+
+}  // namespace
+
+extern "C" int oa_initial_data(oa_handle* h) {
+  using namespace oax;
+  int nlev = 0;
+  if (!h || oa_num_levels(h, &nlev) != OA_OK) return OA_ERR_ARG;
+  int rc = fill_levels(h, nlev, nullptr);
+  if (rc != OA_OK) return rc;
+  const oa_params* pp = nullptr;
+  {
+    double* slab = nullptr;
+    KGrid G{};
+    int nf = 0;
+    bool shift = false, scalar = false;
+    handle_level_slab(h, 0, &slab, &G, &nf, &shift, &scalar, &pp);
+  }
+  if (pp->idata != OA_IDATA_BRILLWAVE && pp->idata != OA_IDATA_SCALARPULSE) return OA_OK;
+  // Brill waves and the scalar pulse: solve the Hamiltonian constraint for
+  // psi on every level (idata_brillwave.f90:117-135,
+  // idata_scalarpulse.f90:131-178), then fill the fields again with it
+  EllSolver* es = nullptr;
+  rc = ell_solve_begin(h, handle_elliptic_exit(h), &es);
+  if (rc != OA_OK) return rc;
+  int steps0 = 0, steps1 = 0;
+  double res = 0.0;
+  rc = ell_solve_run(es, &steps0, &steps1, &res);
+  if (rc == OA_OK) {
+    handle_elliptic_record(h, steps0, steps1, res);
+    std::vector<const double*> psi(static_cast<size_t>(nlev));
+    for (int l = 0; l < nlev; ++l) psi[static_cast<size_t>(l)] = ell_solution(es, l);
+    rc = fill_levels(h, nlev, psi.data());
+  }
+  ell_solve_end(es);
+  return rc;
 }

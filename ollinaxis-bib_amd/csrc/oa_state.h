@@ -62,6 +62,10 @@ struct oa_handle {
   double rhs_avg_ms = 0.0;
   long rhs_launches = 0;
   double point_updates = 0.0;
+  // initial data's elliptic solve: stopping rule and outcome
+  int ell_stationary = 1;            // 1: the all-levels phase may end on the stationary rule
+  int ell_steps[2] = {0, 0};         // iterations: base level alone, all levels
+  double ell_res = 0.0;              // residual of the last iteration
 };
 
 namespace oax {
@@ -80,6 +84,8 @@ bool prepare_finebound(oa_handle* h, int level, double dtw, FbItem* it);   // fa
 // auxiliary(level): *proj / *del say whether the projection / DelDef run
 void prepare_auxiliary(oa_handle* h, int level, ProjItem* pj, bool* proj, DelItem* dd, bool* del);
 void prepare_restrict(oa_handle* h, int level, RsItem* it);
+// box of coarse points a restriction of the fine level Gf onto Gc may write
+void restrict_box(const KGrid& Gf, const KGrid& Gc, int* ic0, int* nic, int* jc0, int* njc);
 void finish_level_step(oa_handle* h, int level);   // advance the level clock, count point updates
 // onestep(level) of every spacetime of `all` that has the level, in lockstep,
 // appended to `plan` (oa_plan.cpp)

@@ -261,22 +261,11 @@ void prepare_auxiliary(oa_handle* h, int level, ProjItem* pj, bool* proj, DelIte
   }
 }
 
-void prepare_restrict(oa_handle* h, int level, RsItem* it) {
-  Level& L = h->L[level];
-  Level& Lc = h->L[level - 1];
-  *it = RsItem{};
-  it->Gf = L.G;
-  it->Gc = Lc.G;
-  it->nf = h->nf;
-  it->interporder = h->p.interporder;
-  it->yf = L.slab[L.cur];
-  it->yc = Lc.slab[Lc.cur];
-  // The coarse points the kernel may restrict onto: r0 in (rmin, rmax), z0 in
-  // (zmin, zmax), with the bounds of restrict_point (restrict.f90:88-162).
-  // The box adds a point of margin on each side and is clipped to the coarse
-  // level; the kernel still tests every point against the exact bounds.
-  const KGrid& Gf = it->Gf;
-  const KGrid& Gc = it->Gc;
+// The coarse points a restriction may write: r0 in (rmin, rmax), z0 in
+// (zmin, zmax), with the bounds of restrict_point (restrict.f90:88-162).
+// The box adds a point of margin on each side and is clipped to the coarse
+// level; the kernels still test every point against the exact bounds.
+void restrict_box(const KGrid& Gf, const KGrid& Gc, int* ic0, int* nic, int* jc0, int* njc) {
   const int g = Gc.g;
   const double deltar = 2.0 * g * Gf.dr + 1.0e-10, deltaz = 2.0 * g * Gf.dz + 1.0e-10;
   const double rminf = (1 - g - 0.5) * Gf.dr, rmaxf = (Gf.Nr - 0.5) * Gf.dr;
@@ -288,12 +277,25 @@ void prepare_restrict(oa_handle* h, int level, RsItem* it) {
   const int ihi = std::min(Gc.Nr, static_cast<int>(std::ceil(rmax / Gc.dr + 0.5)) + 1);
   const int jlo = std::max(1 - g, static_cast<int>(std::floor(zmin / Gc.dz + Gc.zoff)) - 1);
   const int jhi = std::min(Gc.Nz, static_cast<int>(std::ceil(zmax / Gc.dz + Gc.zoff)) + 1);
-  it->ic0 = ilo;
-  it->nic = std::max(1, ihi - ilo + 1);
-  it->jc0 = jlo;
-  it->njc = std::max(1, jhi - jlo + 1);
-  if (it->ic0 + it->nic - 1 > Gc.Nr) it->nic = Gc.Nr - it->ic0 + 1;
-  if (it->jc0 + it->njc - 1 > Gc.Nz) it->njc = Gc.Nz - it->jc0 + 1;
+  *ic0 = ilo;
+  *nic = std::max(1, ihi - ilo + 1);
+  *jc0 = jlo;
+  *njc = std::max(1, jhi - jlo + 1);
+  if (*ic0 + *nic - 1 > Gc.Nr) *nic = Gc.Nr - *ic0 + 1;
+  if (*jc0 + *njc - 1 > Gc.Nz) *njc = Gc.Nz - *jc0 + 1;
+}
+
+void prepare_restrict(oa_handle* h, int level, RsItem* it) {
+  Level& L = h->L[level];
+  Level& Lc = h->L[level - 1];
+  *it = RsItem{};
+  it->Gf = L.G;
+  it->Gc = Lc.G;
+  it->nf = h->nf;
+  it->interporder = h->p.interporder;
+  it->yf = L.slab[L.cur];
+  it->yc = Lc.slab[Lc.cur];
+  restrict_box(it->Gf, it->Gc, &it->ic0, &it->nic, &it->jc0, &it->njc);
   Lc.dirty = true;
   Lc.cons_valid = false;
 }
@@ -718,6 +720,20 @@ extern "C" int oa_norms0d_noalloc(oa_handle* h, int box, int level, int n, const
   return launch_norms_fields_noalloc(h->device, L->G, n, planes, out_dev, h->stream);
 }
 
+extern "C" int oa_set_elliptic_exit(oa_handle* h, int stationary) {
+  if (!h) return OA_ERR_ARG;
+  h->ell_stationary = stationary ? 1 : 0;
+  return OA_OK;
+}
+
+extern "C" int oa_elliptic_stats(const oa_handle* h, int* steps0, int* steps1, double* residual) {
+  if (!h) return OA_ERR_ARG;
+  if (steps0) *steps0 = h->ell_steps[0];
+  if (steps1) *steps1 = h->ell_steps[1];
+  if (residual) *residual = h->ell_res;
+  return OA_OK;
+}
+
 extern "C" int oa_params_footprint(const oa_params* pin, double* bytes) {
   if (!pin || !bytes) return OA_ERR_ARG;
   oa_params p = *pin;
@@ -1074,4 +1090,10 @@ void handle_mark_dirty(oa_handle* h, int level) {
   h->L[level].phi_valid = true;
 }
 void* handle_stream(oa_handle* h) { return h->stream; }
+int handle_elliptic_exit(oa_handle* h) { return h->ell_stationary; }
+void handle_elliptic_record(oa_handle* h, int steps0, int steps1, double residual) {
+  h->ell_steps[0] = steps0;
+  h->ell_steps[1] = steps1;
+  h->ell_res = residual;
+}
 }  // namespace oax

@@ -23,6 +23,7 @@ module oaxis_mod
             oa_initial_aux, oa_coarse_step, oa_evolve, oa_sync, oa_initial_data, &
             oa_last_elapsed_ms, oa_point_updates, oa_params_parse_text, oa_set_kernel_timing, &
             oa_rhs_kernel_stats, oa_norms0d, oa_norms0d_noalloc, oa_params_footprint, &
+            oa_set_elliptic_exit, oa_elliptic_stats, &
             oa_batch_create, oa_batch_destroy, oa_batch_add, oa_batch_remove, oa_batch_size, &
             oa_batch_coarse_step, oa_batch_evolve, oa_batch_sync, oa_batch_last_elapsed_ms, &
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
@@ -300,6 +301,18 @@ module oaxis_mod
       integer(c_int), value :: box, level, n
       type(c_ptr), dimension(*) :: names
       type(c_ptr), value :: out_dev
+    end function
+    ! stopping rule of the initial data's elliptic solve (1: stationary exit too)
+    integer(c_int) function oa_set_elliptic_exit(h, stationary) bind(C, name="oa_set_elliptic_exit")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: stationary
+    end function
+    integer(c_int) function oa_elliptic_stats(h, steps0, steps1, residual) bind(C, name="oa_elliptic_stats")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int) :: steps0, steps1
+      real(c_double) :: residual
     end function
     integer(c_int) function oa_params_footprint(p, bytes) bind(C, name="oa_params_footprint")
       import :: c_int, c_ptr, c_double

@@ -148,8 +148,21 @@ class Spacetime:
         _check(lib().oa_download_field(self._h, 0, level, name.encode(), _dptr(a), a.shape[0]), "download " + name)
         return a
 
-    def initial_data(self):
+    def initial_data(self, stationary_exit=True):
+        """The reference's initial data on every level.  Brill waves and the
+        scalar pulse solve the Hamiltonian constraint (WaveElliptic); with
+        stationary_exit=False its all-levels phase keeps the reference's
+        stopping rule alone (runs to WE_maxiter when the residual stalls)."""
+        _check(lib().oa_set_elliptic_exit(self._h, 1 if stationary_exit else 0), "oa_set_elliptic_exit")
         _check(lib().oa_initial_data(self._h), "oa_initial_data")
+
+    def elliptic_stats(self):
+        """Iterations of the initial data's elliptic solve (base level alone,
+        then all levels) and its last residual."""
+        a, b, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().oa_elliptic_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(r)),
+               "oa_elliptic_stats")
+        return dict(steps0=a.value, steps1=b.value, residual=r.value)
 
     def initial_aux(self):
         _check(lib().oa_initial_aux(self._h), "oa_initial_aux")

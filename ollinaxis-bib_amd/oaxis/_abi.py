@@ -116,6 +116,8 @@ def load_library(path=LIBPATH):
         "oa_norms0d": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, dp]),
         "oa_norms0d_noalloc": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_char_p), ctypes.c_void_p]),
+        "oa_set_elliptic_exit": (ctypes.c_int, [H, ctypes.c_int]),
+        "oa_elliptic_stats": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), dp]),
         "oa_constraints": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int]),
         "oa_ahfind": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(Horizon)]),
         "oa_horizon_shape": (ctypes.c_int, [H, ctypes.c_int, dp, dp, ctypes.c_int, ip]),

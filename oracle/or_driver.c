@@ -137,6 +137,7 @@ or_handle* or_create(const oa_params* pin) {
 
 void or_destroy(or_handle* H) {
   if (!H) return;
+  or_ell_free(H);
   for (int l = 0; l < H->nlev; ++l) level_free(&H->L[l]);
   free(H->L);
   free(H);
@@ -304,21 +305,6 @@ int or_initial_data(or_handle* H) {
         L->psi2[k] = ps * ps;
         L->psi4[k] = ps * ps * ps * ps;
       }
-    } else if (p->idata == OA_IDATA_BRILLWAVE) {
-      /* idata_brillwave.f90:88-105,183-208 (Holz q function); the
-       * Hamiltonian constraint is not solved here: psi = 1 */
-      ALL(L, k) {
-        const double r = L->r[k], z = L->z[k];
-        const double q = p->brill_a0 * r * r * exp(-(r / p->brill_sr0) * (r / p->brill_sr0)
-                                                   - (z / p->brill_sz0) * (z / p->brill_sz0));
-        EV(A)[k] = exp(two * q);
-        EV(B)[k] = EV(A)[k];
-        EV(C)[k] = 0.0;
-        EV(H)[k] = one;
-        EV(lambda)[k] = (EV(A)[k] - EV(H)[k]) / (r * r);
-        EV(Delta_r)[k] = (one - one / EV(A)[k]) / r;
-        EV(Delta_z)[k] = 0.0;
-      }
     } else if (p->idata == OA_IDATA_SCALARPULSE) {   /* idata_scalarpulse.f90:60-103 */
       ALL(L, k) {
         const double r = L->r[k], z = L->z[k], rr = L->rr[k];
@@ -346,6 +332,43 @@ int or_initial_data(or_handle* H) {
       or_diff1r(p, L, EV(scalar_phi), +1, EV(scalar_xi_r));
       or_diff1z(p, L, EV(scalar_phi), +1, EV(scalar_xi_z));
       or_potential(p, L);
+    }
+  }
+  /* Brill waves and the scalar pulse solve the Hamiltonian constraint for
+   * psi on every level (idata_brillwave.f90:117-135,
+   * idata_scalarpulse.f90:131-178) */
+  const int solve = (p->idata == OA_IDATA_BRILLWAVE || p->idata == OA_IDATA_SCALARPULSE);
+  if (solve) or_solve_hamiltonian(H);
+  for (int l = 0; l < H->nlev; ++l) {
+    or_level* L = &H->L[l];
+    if (p->idata == OA_IDATA_BRILLWAVE) {
+      /* idata_brillwave.f90:183-208: psi from the solve, Holz q function */
+      ALL(L, k) {
+        const double r = L->r[k], z = L->z[k];
+        const double q = p->brill_a0 * r * r * exp(-(r / p->brill_sr0) * (r / p->brill_sr0)
+                                                   - (z / p->brill_sz0) * (z / p->brill_sz0));
+        const double ps = L->psi[k];
+        L->psi2[k] = ps * ps;
+        L->psi4[k] = ps * ps * ps * ps;
+        EV(phi)[k] = log(ps);
+        EV(chi)[k] = one / pow(ps, (double)p->chipower);
+        EV(A)[k] = exp(two * q);
+        EV(B)[k] = EV(A)[k];
+        EV(C)[k] = 0.0;
+        EV(H)[k] = one;
+        EV(lambda)[k] = (EV(A)[k] - EV(H)[k]) / (r * r);
+        EV(Delta_r)[k] = (one - one / EV(A)[k]) / r;
+        EV(Delta_z)[k] = 0.0;
+      }
+    } else if (p->idata == OA_IDATA_SCALARPULSE) {
+      /* idata_scalarpulse.f90:185-206: psi from the solve */
+      ALL(L, k) {
+        const double ps = L->psi[k];
+        L->psi2[k] = ps * ps;
+        L->psi4[k] = ps * ps * ps * ps;
+        EV(phi)[k] = log(ps);
+        EV(chi)[k] = one / pow(ps, (double)p->chipower);
+      }
     }
     /* factor the determinant (initial.f90:528-580) */
     if (p->factorhdet) {

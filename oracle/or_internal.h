@@ -109,12 +109,17 @@ typedef struct {
   double *ham, *mom_r, *mom_z, *CDelta_r, *CDelta_z;
 } or_level;
 
+struct or_ell_level;        /* elliptic arrays of one level (or_elliptic.c) */
+
 typedef struct or_handle {
   oa_params p;
   int nlev;                 /* Nl0 + 1                                     */
   double dt0;
   or_level* L;              /* [nlev]                                      */
   double* scratch;          /* one level-sized temporary                   */
+  struct or_ell_level* ell; /* [nlev] while the elliptic solve has run     */
+  int ell_steps[2];         /* iterations: base level alone, all levels    */
+  double ell_res;           /* residual of the last iteration              */
 } or_handle;
 
 /* index of (i,j) with i in [1-g, Nr], j in [1-g, Nz] */
@@ -157,6 +162,8 @@ void or_sources_scalar(const oa_params* p, or_level* L);
 void or_radiative_geometry(const oa_params* p, or_level* L);
 
 double or_interp(const oa_params* p, const or_level* L, const double* f, double r0, double z0);
+int or_solve_hamiltonian(or_handle* H);
+void or_ell_free(or_handle* H);
 void or_finebound(or_handle* H, int level, double dtw);
 void or_restrict(or_handle* H, int level);
 

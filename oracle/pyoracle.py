@@ -59,6 +59,9 @@ def lib():
             "or_ahfind_flat": (ctypes.c_int, [H, ctypes.c_int, ip, dp, dp, dp]),
             "or_ah_integrate": (ctypes.c_int, [H, ctypes.c_double, ctypes.c_double, dp]),
             "or_ah_set_tolerance": (None, [ctypes.c_double, ctypes.c_double]),
+            "or_ell_stats": (None, [H, ip, ip, dp]),
+            "or_set_ell_stationary": (None, [ctypes.c_int]),
+            "or_ell_field": (dp, [H, ctypes.c_int, ctypes.c_char_p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -66,6 +69,12 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def ell_stationary(on):
+    """Stationary exit of the oracle's WaveElliptic solve (off by default:
+    the reference's stopping rule alone; process-wide)."""
+    lib().or_set_ell_stationary(1 if on else 0)
 
 
 def ah_tolerance(rel=1.0e-4, abs_=1.0e-8):
@@ -108,9 +117,21 @@ class Oracle:
         d = self.dims(level)
         return (d["Nr"] + d["ghost"], d["Nz"] + d["ghost"])
 
+    def ell_stats(self):
+        """Iterations of the WaveElliptic solve (base level alone, then all
+        levels; 0 without refinement) and its last residual."""
+        a, b, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        lib().or_ell_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(r))
+        return dict(steps0=a.value, steps1=b.value, residual=r.value)
+
     def view(self, name, level=0, source=False):
-        """Live numpy view (Fortran order) of an evolved or auxiliary array."""
-        fn = lib().or_source if source else lib().or_field
+        """Live numpy view (Fortran order) of an evolved or auxiliary array
+        (or, after the initial data's elliptic solve, ell_u, ell_v, ell_S0,
+        ell_S1, ell_S5)."""
+        if name.startswith("ell_"):
+            fn = lib().or_ell_field
+        else:
+            fn = lib().or_source if source else lib().or_field
         ptr = fn(self._h, level, name.encode())
         if not ptr:
             raise KeyError("oracle has no array %r on level %d" % (name, level))

@@ -262,7 +262,7 @@ def test_scalar_collapse_forms_a_horizon(oaxis_mod, oracle_mod):
     orc.initial_aux()
     st.initial_aux()
     formed = None
-    for step in range(1, 25):
+    for step in range(1, 49):
         orc.coarse_step()
         st.coarse_step()
         if step % 2:
@@ -276,10 +276,12 @@ def test_scalar_collapse_forms_a_horizon(oaxis_mod, oracle_mod):
     st.close()
     orc.close()
     print("collapse: horizon at coarse step %s" % (formed,))
-    assert formed is not None and 4 <= formed[0] <= 20 and 0.1 < formed[1] < 1.0
+    # the elliptic solve makes psi about 2 at the centre: the horizon forms
+    # at about step 36 with mass about 1.8
+    assert formed is not None and 4 <= formed[0] <= 46 and 0.1 < formed[1] < 3.0
     from oaxis.serve import Request, Server
     records, _ = Server(1e12, check_every=0).run(
-        [Request(0, COLLAPSE + "ahfind = .true.\nahfind_every = 2\n", 40, 0.0, "collapse")])
+        [Request(0, COLLAPSE + "ahfind = .true.\nahfind_every = 2\n", 60, 0.0, "collapse")])
     assert records[0].reason == "horizon" and records[0].steps == formed[0]
     assert abs(records[0].horizon["mass"] - formed[1]) < 1e-9
 
@@ -289,8 +291,10 @@ def test_critical_collapse_bisection(oaxis_mod):
     through the serving loop: outcomes are "no horizon" below and "horizon"
     above a threshold, and the bracket shrinks by members - 1 per round."""
     from oaxis.bisection import critical_bisection
+    # with psi from the elliptic solve, pulses above about 0.65 are trapped
+    # from the start (a horizon at t = 0 does not count as formed)
     res = critical_bisection(lambda a: COLLAPSE.replace("scalar_a0 = 0.6", "scalar_a0 = %r" % a),
-                             0.2, 0.8, members=7, rounds=2, steps=48, ahfind_every=2)
+                             0.2, 0.6, members=7, rounds=2, steps=48, ahfind_every=2)
     r1, r2 = res["rounds"]
     print("bisection: %s -> [%.6f, %.6f], %.2f s" % ([(round(a, 3), h) for a, h in zip(r1["amplitudes"], r1["horizon"])],
                                                     res["lo"], res["hi"], res["wall_s"]))
@@ -298,7 +302,7 @@ def test_critical_collapse_bisection(oaxis_mod):
     assert not r1["horizon"][0] and r1["horizon"][-1]
     k = r1["horizon"].index(True)
     assert r2["amplitudes"][0] == r1["amplitudes"][k - 1] and r2["amplitudes"][-1] == r1["amplitudes"][k]
-    assert abs((res["hi"] - res["lo"]) - 0.6 / 36) < 1e-12
+    assert abs((res["hi"] - res["lo"]) - 0.4 / 36) < 1e-12
     assert res["lo"] < 0.6 and res["hi"] > 0.3
 
 
