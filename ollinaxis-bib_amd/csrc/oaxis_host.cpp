@@ -950,12 +950,8 @@ extern "C" int oa_restrict(oa_handle* h, int box, int level) {
 }
 
 extern "C" int oa_advance_time(oa_handle* h, int box, int level) {
-  Level* L = level_of(h, box, level);
-  if (!L) return OA_ERR_ARG;
-  L->t2 = L->t1;
-  L->t1 = L->t;
-  L->s = L->s + 1;
-  L->t = L->t + L->G.dt;
+  if (!level_of(h, box, level)) return OA_ERR_ARG;
+  finish_level_step(h, level);   // the clocks, and the point updates as oa_coarse_step counts them
   return OA_OK;
 }
 

@@ -36,12 +36,17 @@ R2_SCALED = {"lambda": ("A", "H"), "Alambda": ("KTA", "KTH")}
 LOG_SCALED = ("phi",)
 
 
-def compare_state(oaxis, st, orc, params, extra=("KTH", "lambda", "Alambda")):
+def compare_state(oaxis, st, orc, params, extra=("KTH", "lambda", "Alambda"), get=None):
+    """Relative max-norm differences of every evolved array of every level,
+    st (a Spacetime, or any object read through get(name, level)) against the
+    oracle orc."""
+    if get is None:
+        get = st.download
     out = {}
-    for lev in range(st.nlevels):
+    for lev in range(orc.nlevels):
         r = orc.get("r", lev)
         for name in list(oaxis.state_fields(params)) + list(extra):
-            a = st.download(name, lev)
+            a = get(name, lev)
             b = orc.get(name, lev)
             if not np.all(np.isfinite(a)):
                 out[(lev, name)] = float("inf")
@@ -82,3 +87,42 @@ def worst(errs):
         return 0.0, None
     k = max(errs, key=lambda q: errs[q])
     return errs[k], k
+
+
+def run_pair_with_spread(oaxis, pyoracle, text, nsteps, seed=1):
+    """run_pair, plus a third run: the oracle again with every evolved array
+    perturbed by at most one unit in the last place after each coarse step
+    (as two implementations that round differently would).  Returns the
+    GPU-vs-oracle errors, the spread between the two oracle runs (both
+    relative to the oracle array's max norm, per level and array), and the
+    level clocks."""
+    p = oaxis.params_from_text(text)
+    orc = pyoracle.Oracle(p)
+    orc.initial_data()
+    pert = pyoracle.Oracle(p)
+    st = oaxis.Spacetime(p)
+    names = list(oaxis.state_fields(p))
+    for lev in range(st.nlevels):
+        for name in names:
+            a = orc.get(name, lev)
+            st.upload(name, a, lev)
+            pert.set(name, a, lev)
+    rng = np.random.default_rng(seed)
+    orc.initial_aux()
+    pert.initial_aux()
+    st.initial_aux()
+    for _ in range(nsteps):
+        orc.coarse_step()
+        pert.coarse_step()
+        st.coarse_step()
+        for lev in range(pert.nlevels):
+            for name in names:
+                v = pert.get(name, lev)
+                pert.set(name, v * (1.0 + rng.integers(-1, 2, size=v.shape) * 2.0 ** -52), lev)
+    errs = compare_state(oaxis, st, orc, p)
+    spread = compare_state(oaxis, None, orc, p, get=pert.get)
+    times = [(st.time(l), orc.time(l)) for l in range(st.nlevels)]
+    st.close()
+    orc.close()
+    pert.close()
+    return errs, spread, times

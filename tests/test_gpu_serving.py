@@ -45,6 +45,8 @@ def _requests(oaxis):
         Request(3, pulse2, 4, 0.02, "scalar_pulse"),
         Request(4, CASES["schwarzschild_3levels_icn_eqsym"][0], 2, 0.05, "schwarzschild"),
         Request(5, CASES["gauge_pulse_gammadriver2"][0], 7, 0.05, "gauge"),
+        # the deepest requests of the serving mix: 6 levels
+        Request(6, workloads.schwarzschild(0.8, 24, 48, 5, extent=12.8), 3, 0.03, "schwarzschild"),
     ]
 
 

@@ -98,9 +98,21 @@ def test_parse_errors(text, msg):
 
 def test_unstored_reference_keys_are_type_checked():
     # keys of the reference outside the evolution path are accepted and ignored
-    oaxis.params_from_text("ELL_solver = wave\nWE_epsilon = 1.d-8\n")
+    oaxis.params_from_text("outparallel = fileperproc\nNoutput1D = 5\n")
     with pytest.raises(oaxis.OaxisError):
         oaxis.params_from_text("WE_epsilon = tiny\n")
+    with pytest.raises(oaxis.OaxisError):
+        oaxis.params_from_text("Noutput2D = often\n")
+
+
+def test_elliptic_solver_keys_are_stored():
+    p = oaxis.params_from_text("ELL_solver = wave\nWE_epsilon = 1.d-9\nWE_method = rk4\nWE_maxiter = 500\n"
+                               "WE_eta = 0.5\nWE_diss = 0.2\nWE_dtfac = 0.4\nELL_maxiter = 70\n")
+    assert p.ELL_solver == oaxis.ENUMS["OA_ELL_WAVE"] and p.WE_method == oaxis.ENUMS["OA_RK4"]
+    assert (p.WE_epsilon, p.WE_maxiter, p.WE_eta, p.WE_diss, p.WE_dtfac, p.ELL_maxiter) == (1e-9, 500, 0.5, 0.2, 0.4, 70)
+    d = oaxis.default_params()     # param.f90:463-491
+    assert d.ELL_solver == oaxis.ENUMS["OA_ELL_WAVE"] and d.WE_method == oaxis.ENUMS["OA_ICN"]
+    assert (d.WE_epsilon, d.WE_maxiter, d.WE_eta, d.WE_diss, d.WE_dtfac) == (1e-8, 100000, 0.0, 0.1, 0.6)
 
 
 def supported(text):
@@ -116,6 +128,8 @@ def test_supported_configurations():
     assert supported("order = four\n" + size)[0] == 0
     assert supported(SCHWARZ)[0] == 0
     assert supported("order = four\nshift = Gammadriver2\nmattertype = scalar\n" + size)[0] == 0
+    assert supported("order = four\nidata = BrillWave\n" + size)[0] == 0
+    assert supported("order = four\nmattertype = scalar\nidata = scalarpulse\n" + size)[0] == 0
     # the reference's default 10 x 10 grid is below one 16 x 16 tile
     assert supported("order = four\n")[0] != 0
 
@@ -126,6 +140,18 @@ def test_supported_configurations():
     ("order = four\nformulation = z4c\n", "formulation"),
     ("order = four\nmattertype = complex\n", "scalar field"),
     ("order = four\nNrtotal = 4\n", "Nr"),
+    # initial data the library does not build, and solves it does not run
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = kerr\n", "idata"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = testgw\n", "idata"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = checkpoint\n", "idata"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = complexpulse\n", "idata"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = bosonstar\n", "idata"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = rotboson\n", "idata"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = BrillWave\nELL_solver = none\n", "elliptic solver"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = BrillWave\nELL_solver = sor\n", "sor"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nmattertype = scalar\nidata = scalarpulse\nELL_solver = none\n",
+     "elliptic solver"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = scalarpulse\n", "mattertype"),
 ])
 def test_unsupported_configurations_are_refused(text, word):
     rc, msg = supported(text)
