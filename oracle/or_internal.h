@@ -16,8 +16,15 @@
  * symbol, RK4/ICN amplification factors, exact Minkowski stationarity,
  * 4th-order convergence of the static-Schwarzschild residual, bicubic
  * exactness of prolongation/restriction, quadratic exactness of the time
- * interpolation, and an independent symbolic (sympy) derivation of the
- * covariant BSSN / Klein-Gordon right-hand sides evaluated pointwise).
+ * interpolation, an independent symbolic (sympy) derivation of the
+ * covariant BSSN / Klein-Gordon right-hand sides evaluated pointwise, the
+ * radiative boundary as an outgoing spherical wave, the restriction and
+ * fine-boundary point sets derived from coordinates, DelDef as the sympy
+ * connection difference at clock 0, the initial-data closed forms, and the
+ * elliptic solve's discrete equation and Hamiltonian-constraint convergence).
+ * Parity unpinned: the call-order consequences of the first steps (phi-mode
+ * sources in the first substep of a level's second step, the
+ * pre-initialisation dtbeta), which follow the reference's text.
  * The reference Fortran build itself is not available as a further check
  * (it needs Perl-generated sources and stand-ins for missing symbols).
  */

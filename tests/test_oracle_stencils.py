@@ -141,6 +141,34 @@ def test_upwind_side(orc):
     assert np.array_equal(a[i0:Nr - 3 + g - 1, :], b[i0:Nr - 3 + g - 1, :])
 
 
+def test_upwind_z_row_three_minus_ghost(oaxis_mod, oracle_mod):
+    """Without equatorial symmetry the row j = 3-ghost takes the right-biased
+    form (j-1 .. j+3) for beta_z <= 0 as derivadvect.f90 writes it for that
+    row, and the centred form for beta_z > 0: with every row below j = 2-ghost
+    scrambled, the result there is unchanged for beta_z <= 0 and still exact
+    on quartics (the stencil reaches no further than one row down)."""
+    p = oaxis_mod.params_from_text(GRID % ".false.")
+    o = oracle_mod.Oracle(p)
+    r, z = o.coords()
+    g = o.dims()["ghost"]
+    row = 2                                 # array column of j = 3 - g
+    f = r ** 2 * z ** 3 + z ** 4
+    rng = np.random.default_rng(9)
+    f2 = f.copy()
+    f2[:, :row - 1] = rng.normal(size=f2[:, :row - 1].shape)   # j = 1-g scrambled
+    beta = -np.ones_like(r)
+    a = o.stencil("diffadvz", f, beta=beta)
+    b = o.stencil("diffadvz", f2, beta=beta)
+    assert np.array_equal(a[g:, row], b[g:, row])
+    want = 3 * r ** 2 * z ** 2 + 4 * z ** 3
+    assert np.abs(a[g:, row] - want[g:, row]).max() <= 1e-9 * np.abs(want).max()
+    # beta_z > 0 at that row: the centred 4th-order form, which reads j-2
+    c = o.stencil("diffadvz", f2, beta=-beta)
+    assert not np.array_equal(c[g:, row], o.stencil("diffadvz", f, beta=-beta)[g:, row])
+    o.close()
+    del g
+
+
 def test_ko_annihilates_quintics(orc):
     r, z = orc.coords()
     for a, b in monomials(5, 1):
