@@ -33,7 +33,7 @@ def workload_text(name, seed):
     from oaxis import workloads
     if name == "brill":
         return workloads.brill(seed), "brill: BrillWave a~U[1,4] (seed %d), 5 levels of 256x512, ICN, " \
-                                      "1+log + Gammadriver2, psi=1 synthetic" % seed
+                                      "1+log + Gammadriver2, psi from the elliptic solve" % seed
     if name == "plain_small":
         return workloads.plain_small(seed), "plain_small: Schwarzschild M=1, 1 level 128x256, RK4, 1+log"
     if name == "plain_256x512":
@@ -198,21 +198,34 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 8}))
 
 
-def run_serving(args, n, steps_scale, max_grid):
+def run_serving(args, n, steps_scale, max_grid, plain=False):
     """The serving mix through the continuous-batching loop: Poisson
-    arrivals, admission by device footprint, retirement at the final step,
-    on horizon formation (searched every 16 coarse steps) or on dispersal.
-    Returns the loop's summary."""
+    arrivals, admission by device footprint (each request's initial data and
+    elliptic solve prepared on a worker thread while the batch steps),
+    retirement at the final step, on horizon formation (searched every 16
+    coarse steps) or on dispersal.  Returns the loop's summary and, with
+    plain, the summary of the same requests served one spacetime at a time."""
     from oaxis import serve
     grids = tuple(g for g in ((64, 128), (128, 256), (256, 512), (512, 1024)) if g[0] <= max_grid)
     lo, hi = max(1, int(200 * steps_scale)), max(1, int(2000 * steps_scale))
     reqs = serve.serving_mix(args.seed, n, rate=args.serving_rate, grids=grids, steps=(lo, hi))
     srv = serve.Server(memory_budget=args.serving_budget_gb * 1e9, graphs=not args.no_graphs)
-    records, summary = srv.run(reqs)
+
+    def retired(rec):
+        progress("  request %d (%s) retired: %s after %d steps, latency %.2f s"
+                 % (rec.rid, rec.kind, rec.reason, rec.steps, rec.latency))
+
+    records, summary = srv.run(reqs, progress=retired)
     summary["sample"] = ("%d requests of the serving mix (seed %d, Poisson rate %.3g/s, grids up to %dx%d, "
-                         "1-6 levels, %d-%d coarse steps, horizon search every 16), memory budget %.0f GB"
-                         % (n, args.seed, args.serving_rate, max_grid, 2 * max_grid, lo, hi, args.serving_budget_gb))
-    return summary
+                         "1-6 levels, %d-%d coarse steps (the mix's 200-2000 scaled by %g), horizon search "
+                         "every 16), memory budget %.0f GB"
+                         % (n, args.seed, args.serving_rate, max_grid, 2 * max_grid, lo, hi, steps_scale,
+                            args.serving_budget_gb))
+    if not plain:
+        return summary
+    progress("serving mix: %.3g updates/s batched; the same requests one at a time" % summary["updates_per_s"])
+    _, plain_summary = srv.run_plain(reqs, progress=retired)
+    return summary, plain_summary
 
 
 def horizon_probe(with_oracle):
@@ -258,12 +271,33 @@ def run_bisection(args):
                              rounds=max(1, args.steps), steps=48, ahfind_every=2)
     print(json.dumps({"metric": METRIC, "value": res["point_updates"] / res["wall_s"], "unit": "grid-point updates/s",
                       "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
-                      "data": "synthetic (central scalar pulses; no elliptic solve)",
+                      "data": "synthetic (central scalar pulses; psi from the elliptic solve)",
                       "config": {"workload": "critical-collapse bisection (reduced): %d amplitudes per round, "
                                              "%d round(s), 2 levels of 32x32, eqsym" % (args.members, len(res["rounds"])),
                                  "bracket": [res["lo"], res["hi"]], "wall_s": res["wall_s"],
                                  "rounds": [{k: r[k] for k in ("amplitudes", "horizon", "steps", "wall_s", "p50_s",
                                                                 "p99_s")} for r in res["rounds"]]}}))
+
+
+def prepare_spacetimes(texts, threads=8):
+    """Spacetimes with their initial data (and its elliptic solve) and the
+    initial auxiliary pass, prepared on worker threads: each runs on its own
+    stream, so the many small kernels of the solves overlap on the device."""
+    import oaxis
+    from concurrent.futures import ThreadPoolExecutor
+
+    def make(text):
+        st = oaxis.Spacetime(oaxis.params_from_text(text))
+        st.initial_data()
+        st.initial_aux()
+        return st
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
+        return list(pool.map(make, texts))
+
+
+def progress(msg):
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
 
 
 def run_batched(args):
@@ -274,19 +308,19 @@ def run_batched(args):
     import oaxis
     from oaxis import workloads
     texts = [workloads.scalar_pulse(args.seed, i) for i in range(args.members)]
-    sts = []
-    for t in texts:
-        st = oaxis.Spacetime(oaxis.params_from_text(t))
-        st.initial_data()
-        st.initial_aux()
-        sts.append(st)
+    t_prep = time.perf_counter()
+    sts = prepare_spacetimes(texts)
+    t_prep = time.perf_counter() - t_prep
+    progress("%d members prepared (initial data with the elliptic solve) in %.1f s" % (len(sts), t_prep))
     nfields = workloads.evolved_count(True, True)
     b = oaxis.Batch(sts)
     if args.no_graphs:
         b.set_graphs(False)
-    # warm-up, long enough for the schedule autotuner to finish (it times
-    # each schedule over steady-state steps)
-    b.evolve(max(args.warmup, TUNE_STEPS))
+    # untimed: the schedule autotuner's steps (it times each schedule over
+    # steady-state steps), then the requested warm-up
+    b.evolve(TUNE_STEPS)
+    if args.warmup > 0:
+        b.evolve(args.warmup)
     b.sync()
     u0 = b.point_updates()
     t0 = time.perf_counter()
@@ -306,10 +340,13 @@ def run_batched(args):
     # the plain path: every member alone, one after the other
     plain_updates = 0.0
     plain_wall = 0.0
+    progress("batched: %.3g updates/s; timing the plain path" % (updates / wall))
     for st in sts:
         if args.no_graphs:
             st.set_graphs(False)
         st.evolve(TUNE_STEPS)          # untimed: the member's own schedule autotuning
+        if args.warmup > 0:
+            st.evolve(args.warmup)
         p0 = st.point_updates()
         st.sync()
         t1 = time.perf_counter()
@@ -332,16 +369,17 @@ def run_batched(args):
         "unit": "grid-point updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": max(args.warmup, TUNE_STEPS),
+        "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded par generator, seed %d)" % args.seed,
+        "data": "synthetic (seeded par generator, seed %d; psi from the elliptic solve)" % args.seed,
         "config": {"workload": "batched scalar collapse: %d scalar pulses A~U[.01,.05], r0~U[3,6], s~U[.5,1.5] "
                                "(seed %d), grids %s, 2-4 levels, RK4, 1+log + Gammadriver2"
                                % (args.members, args.seed, " ".join("%dx%d" % g for g in grids)),
+                   "tuning_steps": TUNE_STEPS, "prepare_s": t_prep,
                    "path": "batched (one launch per kernel kind over all members)",
                    "members": args.members, "levels_per_member": levels, "evolved_fields": nfields,
                    "integrator": "rk4", "point_updates": updates, "device_ms": device_ms,
@@ -357,12 +395,16 @@ def run_batched(args):
         "fp64": fp64_roofline(args.workload, rhs_points / rhs_n if rhs_n else 0.0,
                               rhs_ms / rhs_n if rhs_n else 0.0),
     }
+    progress("plain: %.3g updates/s; horizon probe" % plain_value)
     rec["horizon"] = horizon_probe(not args.no_cpu_baseline)
     if args.serving_requests > 0:
-        sv = run_serving(args, args.serving_requests, args.serving_steps_scale, 256)
+        progress("serving mix: %d requests" % args.serving_requests)
+        sv, pl = run_serving(args, args.serving_requests, args.serving_steps_scale, 512, plain=True)
         rec["latency"] = {"p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "requests": sv["requests"],
                           "updates_per_s": sv["updates_per_s"], "wall_s": sv["wall_s"],
-                          "retired": sv["reasons"], "sample": sv["sample"]}
+                          "retired": sv["reasons"], "sample": sv["sample"],
+                          "plain_updates_per_s": pl["updates_per_s"], "plain_wall_s": pl["wall_s"],
+                          "batched_vs_plain": sv["updates_per_s"] / pl["updates_per_s"]}
     if not args.no_cpu_baseline:
         sample = texts[:8]
         rec["cpu_baseline"] = cpu_baseline_record(
@@ -414,12 +456,15 @@ def main():
         print(json.dumps({"horizon": horizon_probe(not args.no_cpu_baseline)}))
         return
     if args.workload == "serving":
-        # the full serving mix (grids up to 512x1024, 200-2000 steps scaled by --serving-steps-scale)
-        sv = run_serving(args, args.serving_requests, args.serving_steps_scale, 512)
+        # the full serving mix (grids up to 512x1024, 200-2000 steps scaled by
+        # --serving-steps-scale), then the same requests on the plain path
+        sv, pl = run_serving(args, args.serving_requests, args.serving_steps_scale, 512, plain=True)
         print(json.dumps({"metric": METRIC, "value": sv["updates_per_s"], "unit": "grid-point updates/s",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
-                          "data": "synthetic (seeded par generator, seed %d)" % args.seed,
-                          "config": {"workload": "serving mix", "sample": sv["sample"]},
+                          "data": "synthetic (seeded par generator, seed %d; psi from the elliptic solve)" % args.seed,
+                          "config": {"workload": "serving mix", "sample": sv["sample"],
+                                     "plain_value": pl["updates_per_s"], "plain_wall_s": pl["wall_s"],
+                                     "batched_vs_plain": sv["updates_per_s"] / pl["updates_per_s"]},
                           "latency": {k: sv[k] for k in ("p50_s", "p99_s", "requests", "wall_s", "reasons")}}))
         return
 
@@ -436,7 +481,9 @@ def main():
     st.initial_aux()
     if args.no_graphs:
         st.set_graphs(False)
-    st.evolve(max(args.warmup, TUNE_STEPS))
+    st.evolve(TUNE_STEPS)              # untimed: the schedule autotuner's steps
+    if args.warmup > 0:
+        st.evolve(args.warmup)
     u0 = st.point_updates()
     st.sync()
     t0 = time.perf_counter()
@@ -469,14 +516,14 @@ def main():
         "unit": "grid-point updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": max(args.warmup, TUNE_STEPS),
+        "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded par generator, seed %d)" % args.seed,
-        "config": {"workload": desc, "path": "plain (one spacetime)", "levels": st.nlevels,
+        "config": {"workload": desc, "path": "plain (one spacetime)", "levels": st.nlevels, "tuning_steps": TUNE_STEPS,
                    "Nrtotal": params.Nrtotal, "Nztotal": params.Nztotal, "evolved_fields": nfields,
                    "integrator": "rk4" if params.integrator == 1 else "icn",
                    "point_updates": updates, "device_ms": device_ms, "schedule": schedule},

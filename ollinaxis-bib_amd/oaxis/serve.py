@@ -20,10 +20,17 @@ coarse steps: min alpha on the finest level below ``alpha_collapse``, for
 requests whose lapse starts above it).  The reference has no serving loop;
 the retirement criteria other than the step count are defined here.
 
+Admission prepares a request (device allocation, initial data with its
+elliptic solve, the initial auxiliary pass) on a host worker thread and the
+request's own stream, while the batch keeps stepping on its stream; the
+prepared spacetime joins the batch at the next step boundary.
+
 Completion latency is retirement time minus arrival time; run() returns one
-record per request and the aggregate throughput.
+record per request and the aggregate throughput.  run_plain() serves the
+same requests one spacetime at a time, for comparison.
 """
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -59,9 +66,10 @@ class Record:
 
 class Server:
     def __init__(self, memory_budget, max_members=256, check_every=16, alpha_collapse=0.05,
-                 disperse_frac=0.01, on_retire=None, graphs=True):
+                 disperse_frac=0.01, on_retire=None, graphs=True, prepare_threads=4):
         import oaxis
         self.oaxis = oaxis
+        self.prepare_threads = max(1, int(prepare_threads))
         self.budget = float(memory_budget)
         self.max_members = int(max_members)
         self.check_every = int(check_every)
@@ -70,13 +78,27 @@ class Server:
         self.on_retire = on_retire       # callback(request, spacetime, record) before the state is freed
         self.graphs = bool(graphs)       # coarse steps as captured graphs (False: kernel by kernel)
 
-    def _admit(self, req, now, batch, active, used):
+    def _prepare(self, req):
+        """Device state of a request, ready to join the batch (runs on a
+        worker thread: the library calls release the interpreter lock and use
+        the spacetime's own stream)."""
         oaxis = self.oaxis
         p = oaxis.params_from_text(req.text)
-        fp = oaxis.footprint(p)
         st = oaxis.Spacetime(p)
-        st.initial_data()
-        st.initial_aux()
+        try:
+            st.initial_data()
+            st.initial_aux()
+        except Exception:
+            st.close()
+            raise
+        return p, st
+
+    def _admit(self, req, now, batch, active, used, prepared=None):
+        oaxis = self.oaxis
+        if prepared is None:
+            prepared = self._prepare(req)
+        p, st = prepared
+        fp = oaxis.footprint(p)
         rec = Record(req.rid, req.kind, req.arrival, admitted=now, footprint=fp)
         scalar = bool(p.mattertype & oaxis.ENUMS["OA_MATTER_SCALAR"])
         phi0 = st.norms0d("scalar_phi", 0)["nm1"] if scalar else 0.0
@@ -95,7 +117,7 @@ class Server:
                            ah_t0=ah and p.ahafter <= 0.0,
                            ah_every=max(1, int(p.ahfind_every)), ah_after=float(p.ahafter),
                            dt0=st.level_info(0)["dt"], u0=st.point_updates()))
-        return used + fp
+        return fp
 
     def _initial_searches(self, batch, active):
         """The t = 0 horizon searches of the members admitted since the last
@@ -147,34 +169,45 @@ class Server:
             return "failed"          # a NaN in the state: the request retires with that status
         return None
 
-    def run(self, requests, clock=time.perf_counter):
+    def run(self, requests, clock=time.perf_counter, progress=None):
         """Serve ``requests`` (arrival times relative to the start); returns
-        (records, summary)."""
+        (records, summary).  progress(record) is called at each retirement."""
         oaxis = self.oaxis
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active, done = [], []
+        preparing = []            # (request, admission time, footprint, future), in admission order
         used = 0.0
         batch = oaxis.Batch()
         # the membership changes too often for the schedule autotuner to
         # settle: a fixed schedule (graphs unless switched off; RHS blocks
         # renumbered per XCD)
         batch.set_graphs(self.graphs)
+        pool = ThreadPoolExecutor(max_workers=self.prepare_threads)
         t0 = clock()
         total_updates = 0.0
         try:
-            while pending or active:
+            while pending or active or preparing:
                 now = clock() - t0
-                # admit what has arrived, while memory and slots allow
-                while (pending and pending[0].arrival <= now and len(active) < self.max_members):
+                # admit what has arrived, while memory and slots allow: the
+                # request is prepared on a worker thread meanwhile
+                while (pending and pending[0].arrival <= now
+                       and len(active) + len(preparing) < self.max_members):
                     fp = oaxis.footprint(oaxis.params_from_text(pending[0].text))
-                    if active and used + fp > self.budget:
+                    if (active or preparing) and used + fp > self.budget:
                         break
                     if fp > self.budget:
                         raise MemoryError("request %d needs %.3g B, more than the budget" % (pending[0].rid, fp))
-                    used = self._admit(pending.pop(0), now, batch, active, used)
+                    req = pending.pop(0)
+                    used += fp
+                    preparing.append((req, now, fp, pool.submit(self._prepare, req)))
                     now = clock() - t0
+                # prepared requests join at this step boundary, in admission order
+                while preparing and (preparing[0][3].done() or not active):
+                    req, t_adm, fp, fut = preparing.pop(0)
+                    self._admit(req, t_adm, batch, active, used, prepared=fut.result())
                 if not active:
-                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                    if pending and not preparing:
+                        time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                     continue
                 for a in self._initial_searches(batch, active):
                     rec = self._retire(a, "failed", clock() - t0, batch)
@@ -187,31 +220,7 @@ class Server:
                 batch.coarse_step()
                 for a in active:
                     a["steps"] += 1
-                retire = []
-                due = []
-                for a in active:
-                    if a["steps"] >= a["req"].steps:
-                        retire.append((a, "final_time"))
-                        continue
-                    if self.check_every > 0 and a["steps"] % self.check_every == 0:
-                        reason = self._check(a)
-                        if reason:
-                            retire.append((a, reason))
-                            continue
-                    if (a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0
-                            and a["steps"] * a["dt0"] >= a["ah_after"]):
-                        due.append(a)
-                if due:
-                    try:
-                        found = batch.ahfind([(a["st"], 0) for a in due])
-                    except oaxis.OaxisError:
-                        found = None
-                    for q, a in enumerate(due):
-                        if found is None:
-                            retire.append((a, "failed"))
-                        elif found[q]["found"]:
-                            a["rec"].horizon = found[q]
-                            retire.append((a, "horizon"))
+                retire = self._due_retirements(batch, active)
                 if retire:
                     batch.sync()
                     now = clock() - t0
@@ -221,17 +230,120 @@ class Server:
                         used -= rec.footprint
                         active.remove(a)
                         done.append(rec)
+                        if progress is not None:
+                            progress(rec)
         finally:
+            for _, _, _, fut in preparing:
+                try:
+                    fut.result()[1].close()
+                except Exception:
+                    pass
+            pool.shutdown(wait=True)
             for a in active:
                 a["st"].close()
             batch.close()
-        wall = clock() - t0
+        return self._summary(done, clock() - t0, total_updates)
+
+    def _due_retirements(self, batch, active):
+        """Members to retire after a coarse step: final step, a check every
+        check_every steps (lapse collapse, dispersal, NaN), and the horizon
+        searches due at this step, run as one lockstep batch."""
+        oaxis = self.oaxis
+        retire = []
+        due = []
+        for a in active:
+            if a["steps"] >= a["req"].steps:
+                retire.append((a, "final_time"))
+                continue
+            if self.check_every > 0 and a["steps"] % self.check_every == 0:
+                reason = self._check(a)
+                if reason:
+                    retire.append((a, reason))
+                    continue
+            if (a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0
+                    and a["steps"] * a["dt0"] >= a["ah_after"]):
+                due.append(a)
+        if due:
+            try:
+                found = batch.ahfind([(a["st"], 0) for a in due])
+            except oaxis.OaxisError:
+                found = None
+            for q, a in enumerate(due):
+                if found is None:
+                    retire.append((a, "failed"))
+                elif found[q]["found"]:
+                    a["rec"].horizon = found[q]
+                    retire.append((a, "horizon"))
+        return retire
+
+    @staticmethod
+    def _summary(done, wall, total_updates):
         lat = np.array([r.latency for r in done]) if done else np.zeros(1)
         summary = dict(requests=len(done), wall_s=wall, point_updates=total_updates,
                        updates_per_s=total_updates / wall if wall > 0 else 0.0,
                        p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                        reasons={k: sum(1 for r in done if r.reason == k) for k in set(r.reason for r in done)})
         return sorted(done, key=lambda r: r.rid), summary
+
+    def run_plain(self, requests, clock=time.perf_counter, progress=None):
+        """The same requests one spacetime at a time (the plain path): each is
+        prepared, evolved with the same retirement rules and searches, and
+        retired before the next starts; arrival times are ignored (every
+        request is taken as waiting from the start).  Returns (records,
+        summary) like run()."""
+        done = []
+        total_updates = 0.0
+        t0 = clock()
+        for req in sorted(requests, key=lambda r: (r.arrival, r.rid)):
+            batch = self.oaxis.Batch()
+            batch.set_graphs(self.graphs)
+            active = []
+            try:
+                self._admit(req, clock() - t0, batch, active, 0.0)
+                batch.remove(active[0]["st"])      # the plain path: the spacetime alone
+                a = active[0]
+                st = a["st"]
+                if a["ah_t0"]:
+                    a["ah_t0"] = False
+                    if st.ahfind(0)["found"]:
+                        a["watch"] = False
+                reason = None
+                while reason is None:
+                    st.coarse_step()
+                    a["steps"] += 1
+                    reason = self._plain_retirement(a)
+                rec = a["rec"]
+                rec.retired = clock() - t0
+                rec.arrival = 0.0
+                rec.reason = reason
+                rec.steps = a["steps"]
+                rec.point_updates = st.point_updates() - a["u0"]
+                total_updates += rec.point_updates
+                done.append(rec)
+                st.close()
+                if progress is not None:
+                    progress(rec)
+            finally:
+                batch.close()
+        return self._summary(done, clock() - t0, total_updates)
+
+    def _plain_retirement(self, a):
+        st = a["st"]
+        if a["steps"] >= a["req"].steps:
+            return "final_time"
+        if self.check_every > 0 and a["steps"] % self.check_every == 0:
+            reason = self._check(a)
+            if reason:
+                return reason
+        if a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0 and a["steps"] * a["dt0"] >= a["ah_after"]:
+            try:
+                h = st.ahfind(0)
+            except self.oaxis.OaxisError:
+                return "failed"
+            if h["found"]:
+                a["rec"].horizon = h
+                return "horizon"
+        return None
 
 
 def serving_mix(seed, n, rate, grids=((64, 128), (128, 256), (256, 512), (512, 1024)), max_levels=6,

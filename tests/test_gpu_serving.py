@@ -77,10 +77,9 @@ def test_serving_results_equal_alone(oaxis_mod):
     srv = Server(budget, check_every=0, on_retire=keep)
     orig_admit = srv._admit
 
-    def admit(req, now, batch, active, used):
-        u = orig_admit(req, now, batch, active, used)
-        peak.append(u)
-        return u
+    def admit(req, now, batch, active, used, prepared=None):
+        peak.append(used)            # the footprints admitted so far, this request's included
+        return orig_admit(req, now, batch, active, used, prepared)
 
     srv._admit = admit
     print("serving test: %d requests, budget %.3g B" % (len(reqs), budget), flush=True)
@@ -117,6 +116,21 @@ scalar_z0 = 0.0
 scalar_sr0 = 1.0
 scalar_sz0 = 1.0
 """
+
+
+def test_plain_serving_matches_batched_outcomes(oaxis_mod):
+    """run_plain serves the same requests one spacetime at a time: every
+    request ends with the same reason, step count and point updates as in
+    the continuous batch."""
+    from oaxis.serve import Server
+    reqs = _requests(oaxis_mod)
+    srv = Server(1e12, check_every=0)
+    rec_b, sum_b = srv.run(reqs)
+    rec_p, sum_p = srv.run_plain(reqs)
+    print("serving: batched %.3g updates/s, plain %.3g updates/s" % (sum_b["updates_per_s"], sum_p["updates_per_s"]))
+    for a, b in zip(rec_b, rec_p):
+        assert (a.rid, a.reason, a.steps, a.point_updates) == (b.rid, b.reason, b.steps, b.point_updates)
+    assert sum_b["point_updates"] == sum_p["point_updates"]
 
 
 def test_serving_retires_a_dispersed_pulse(oaxis_mod):
