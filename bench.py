@@ -92,30 +92,36 @@ def fp64_roofline(workload, points_per_launch, avg_launch_ms):
             "frac": tflops / FP64_PEAK_TFLOPS, "flop_per_point": per_point}
 
 
-def cpu_baseline(texts, min_seconds, threads=1):
+def cpu_baseline(texts, min_seconds, threads=1, states=None):
     """The oracle (plain C restatement) on a bounded sample: coarse steps of
     the given configurations until min_seconds of wall time have passed.
     With threads > 1 each worker thread owns instances of its own (thread t
     takes texts[t % len(texts)]) and steps them independently; the oracle's
     calls release the GIL, so the threads run on separate cores, like
-    independent runs of the reference, one per core.  Returns (aggregate
-    updates/s, seconds, coarse steps)."""
+    independent runs of the reference, one per core.  states[k], when given,
+    is the initial state of texts[k] ({(level, name): array}, e.g. the
+    device's, so the oracle need not repeat the elliptic solve).  Returns
+    (aggregate updates/s, seconds, coarse steps)."""
     import threading
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oaxis
     import pyoracle
 
-    def make(text):
-        orc = pyoracle.Oracle(oaxis.params_from_text(text))
-        orc.initial_data()
+    def make(k):
+        orc = pyoracle.Oracle(oaxis.params_from_text(texts[k]))
+        if states is not None:
+            for (lev, name), a in states[k].items():
+                orc.set(name, a, lev)
+        else:
+            orc.initial_data()
         orc.initial_aux()
         d = orc.dims(0)
         return orc, sum(2 ** l for l in range(orc.nlevels)) * d["Nr"] * d["Nz"]
 
     if threads <= 1:
-        owned = [[text for text in texts]]
+        owned = [list(range(len(texts)))]
     else:
-        owned = [[texts[t % len(texts)]] for t in range(threads)]
+        owned = [[t % len(texts)] for t in range(threads)]
     orcs = [None] * len(owned)
     tally = [(0.0, 0)] * len(owned)
     start = threading.Barrier(len(owned) + 1)
@@ -124,7 +130,7 @@ def cpu_baseline(texts, min_seconds, threads=1):
 
     def worker(t):
         try:
-            orcs[t] = [make(text) for text in owned[t]]
+            orcs[t] = [make(k) for k in owned[t]]
         except Exception as e:          # let the other threads and the caller go on
             errors.append(e)
             start.abort()
@@ -172,15 +178,16 @@ def host_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline_record(texts, seconds, what):
+def cpu_baseline_record(texts, seconds, what, states=None):
     """cpu_baseline of the bench line: the oracle on all host cores (value),
     with the one-core figure beside it."""
-    v1, s1, n1 = cpu_baseline(texts, seconds, 1)
+    progress("CPU baseline: the oracle on one core, then on %d" % host_threads())
+    v1, s1, n1 = cpu_baseline(texts, seconds, 1, states)
     nt = host_threads()
-    vn, sn, nn = cpu_baseline(texts, seconds, nt)
+    vn, sn, nn = cpu_baseline(texts, seconds, nt, states)
     return {"value": vn, "unit": "grid-point updates/s", "cores": nt, "kind": "port",
             "sample": "%s on the oracle (C restatement), %d threads each stepping its own instance, "
-                      "%d coarse steps in %.1f s" % (what, nt, nn, sn),
+                      "%d coarse steps in %.1f s (evolution only)" % (what, nt, nn, sn),
             "one_core": {"value": v1, "sample": "%d coarse steps, 1 thread, %.1f s" % (n1, s1)}}
 
 
@@ -312,6 +319,9 @@ def run_batched(args):
     sts = prepare_spacetimes(texts)
     t_prep = time.perf_counter() - t_prep
     progress("%d members prepared (initial data with the elliptic solve) in %.1f s" % (len(sts), t_prep))
+    # the initial states of the CPU baseline's sample, from the device
+    cpu_states = [{(lev, name): st.download(name, lev) for lev in range(st.nlevels)
+                   for name in oaxis.state_fields(st.params)} for st in sts[:8]]
     nfields = workloads.evolved_count(True, True)
     b = oaxis.Batch(sts)
     if args.no_graphs:
@@ -408,7 +418,8 @@ def run_batched(args):
     if not args.no_cpu_baseline:
         sample = texts[:8]
         rec["cpu_baseline"] = cpu_baseline_record(
-            sample, args.cpu_seconds, "members 0-%d of the same workload" % (len(sample) - 1))
+            sample, args.cpu_seconds, "members 0-%d of the same workload (their initial states taken from the "
+            "device)" % (len(sample) - 1), cpu_states)
     print(json.dumps(rec))
 
 
@@ -479,6 +490,8 @@ def main():
     st = oaxis.Spacetime(params)
     st.initial_data()
     st.initial_aux()
+    cpu_state = {(lev, name): st.download(name, lev) for lev in range(st.nlevels)
+                 for name in oaxis.state_fields(params)}
     if args.no_graphs:
         st.set_graphs(False)
     st.evolve(TUNE_STEPS)              # untimed: the schedule autotuner's steps
@@ -535,7 +548,8 @@ def main():
     }
     st.close()
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline_record([text], args.cpu_seconds, "the same workload")
+        rec["cpu_baseline"] = cpu_baseline_record([text], args.cpu_seconds, "the same workload (initial state "
+                                                  "taken from the device)", [cpu_state])
     print(json.dumps(rec))
 
 
