@@ -286,21 +286,49 @@ def run_bisection(args):
                                                                 "p99_s")} for r in res["rounds"]]}}))
 
 
-def prepare_spacetimes(texts, threads=8):
+def prepare_spacetimes(texts, threads=8, graphs=True):
     """Spacetimes with their initial data (and its elliptic solve) and the
     initial auxiliary pass, prepared on worker threads: each runs on its own
-    stream, so the many small kernels of the solves overlap on the device."""
+    stream, so the many small kernels of the solves overlap on the device.
+    graphs=False launches everything kernel by kernel, the solve included."""
     import oaxis
     from concurrent.futures import ThreadPoolExecutor
 
     def make(text):
         st = oaxis.Spacetime(oaxis.params_from_text(text))
+        if not graphs:
+            st.set_graphs(False)
         st.initial_data()
         st.initial_aux()
         return st
 
     with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
         return list(pool.map(make, texts))
+
+
+def initial_spacetimes(texts, args):
+    """The prepared spacetimes of a workload.  With --initial-states DIR they
+    are restored from the checkpoints of an earlier run in DIR (one file per
+    member, same parameters), or prepared and then saved there: a profiled
+    run can then start from the same states without running the initial
+    data and its elliptic solve itself."""
+    import oaxis
+    d = args.initial_states
+    paths = [os.path.join(d, "member_%03d.ckpt" % k) for k in range(len(texts))] if d else []
+    if paths and all(os.path.exists(p) for p in paths):
+        sts = [oaxis.load_checkpoint(p) for p in paths]
+        for k, (st, text) in enumerate(zip(sts, texts)):
+            if bytes(st.params) != bytes(oaxis.params_from_text(text)):
+                raise SystemExit("%s holds another workload's member %d" % (d, k))
+            if args.no_graphs:
+                st.set_graphs(False)
+        return sts, "restored from %s" % d
+    sts = prepare_spacetimes(texts, threads=args.prepare_threads, graphs=not args.no_graphs)
+    if paths:
+        os.makedirs(d, exist_ok=True)
+        for p, st in zip(paths, sts):
+            oaxis.save_checkpoint(st, p)
+    return sts, "prepared (initial data with the elliptic solve)"
 
 
 def progress(msg):
@@ -316,9 +344,9 @@ def run_batched(args):
     from oaxis import workloads
     texts = [workloads.scalar_pulse(args.seed, i) for i in range(args.members)]
     t_prep = time.perf_counter()
-    sts = prepare_spacetimes(texts)
+    sts, how = initial_spacetimes(texts, args)
     t_prep = time.perf_counter() - t_prep
-    progress("%d members prepared (initial data with the elliptic solve) in %.1f s" % (len(sts), t_prep))
+    progress("%d members %s in %.1f s" % (len(sts), how, t_prep))
     # the initial states of the CPU baseline's sample, from the device
     cpu_states = [{(lev, name): st.download(name, lev) for lev in range(st.nlevels)
                    for name in oaxis.state_fields(st.params)} for st in sts[:8]]
@@ -442,7 +470,12 @@ def main():
                     help="scale of the 200-2000 coarse steps per request in the serving sample")
     ap.add_argument("--serving-rate", type=float, default=20.0, help="Poisson arrival rate (1/s)")
     ap.add_argument("--serving-budget-gb", type=float, default=200.0, help="device memory for admitted requests")
-    ap.add_argument("--no-graphs", action="store_true", help="launch coarse steps kernel by kernel")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch coarse steps and elliptic solves kernel by kernel")
+    ap.add_argument("--prepare-threads", type=int, default=8,
+                    help="host threads preparing the batched members' initial data")
+    ap.add_argument("--initial-states", default="", metavar="DIR",
+                    help="restore the prepared members from checkpoints in DIR, or save them there")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -487,13 +520,9 @@ def main():
     scalar = bool(params.mattertype & oaxis.ENUMS["OA_MATTER_SCALAR"])
     nfields = workloads.evolved_count(shift, scalar)
 
-    st = oaxis.Spacetime(params)
-    st.initial_data()
-    st.initial_aux()
+    (st,), _ = initial_spacetimes([text], args)
     cpu_state = {(lev, name): st.download(name, lev) for lev in range(st.nlevels)
                  for name in oaxis.state_fields(params)}
-    if args.no_graphs:
-        st.set_graphs(False)
     st.evolve(TUNE_STEPS)              # untimed: the schedule autotuner's steps
     if args.warmup > 0:
         st.evolve(args.warmup)

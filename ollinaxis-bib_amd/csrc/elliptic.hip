@@ -622,26 +622,32 @@ struct EllSolver {
     return launched("stopping rule");
   }
 
-  // one phase: coarse steps over levels 0..nlmax until the rule fires
+  // one phase: coarse steps over levels 0..nlmax until the rule fires.  The
+  // coarse step is captured once as a graph and replayed, unless the handle's
+  // caller turned graphs off (oa_set_graphs): then every step is launched
+  // kernel by kernel, with the same kernels in the same order.
   int phase(int nlmax, int stationary_rule, int* steps, double* gres) {
     hipLaunchKernelGGL(k_ell_reset, dim3(1), dim3(64), 0, st, ctl, stationary_rule, clocks, nlev);
     ELLRC(launched("reset"));
+    const bool use_graph = h->plan.graphs || h->plan.tuner.enabled;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    ELLHIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
-    const int rc = coarse_step(nlmax);
-    hipGraph_t captured = nullptr;
-    const hipError_t ec = hipStreamEndCapture(st, &captured);
-    if (rc != OA_OK) {
-      if (captured) (void)hipGraphDestroy(captured);
-      return rc;
-    }
-    ELLHIP(ec, "graph capture");
-    graph = captured;
-    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ei != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      ELLHIP(ei, "graph instantiation");
+    if (use_graph) {
+      ELLHIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+      const int rc = coarse_step(nlmax);
+      hipGraph_t captured = nullptr;
+      const hipError_t ec = hipStreamEndCapture(st, &captured);
+      if (rc != OA_OK) {
+        if (captured) (void)hipGraphDestroy(captured);
+        return rc;
+      }
+      ELLHIP(ec, "graph capture");
+      graph = captured;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        ELLHIP(ei, "graph instantiation");
+      }
     }
     int status = OA_OK;
     const int maxiter = h->p.WE_maxiter;
@@ -650,6 +656,10 @@ struct EllSolver {
       // at least 100 steps are always taken; then chunks of 64
       const int chunk = (launched_steps == 0) ? 100 : 64;
       for (int k = 0; k < chunk && status == OA_OK; ++k) {
+        if (!use_graph) {
+          status = coarse_step(nlmax);
+          continue;
+        }
         const hipError_t e = hipGraphLaunch(exec, st);
         if (e != hipSuccess) {
           oa_set_error(std::string("elliptic solve: graph launch: ") + hipGetErrorString(e));
@@ -672,8 +682,8 @@ struct EllSolver {
         break;
       }
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (status != OA_OK) return status;
     *steps = host_ctl->step;
     *gres = host_ctl->gres;

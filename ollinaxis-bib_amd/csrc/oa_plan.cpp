@@ -80,9 +80,29 @@ void drop_graph(Plan::GraphSlot& g) {
   g.graph = nullptr;
   g.sig.clear();
   g.bytes = 0;
+  g.dev = nullptr;
 }
 
 }  // namespace
+
+// A larger device arena for the work items.  Every earlier launch on the
+// stream has finished with the old arena once the stream is idle; the
+// captured graphs of both slots copy into the old arena and launch kernels
+// that read it, so they are destroyed with it (no exec outlives the memory
+// it names) and are captured again on their next use.
+int Plan::grow_arena(hipStream_t stream, size_t bytes) {
+  if (dev) {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipFree(dev));
+  }
+  for (GraphSlot& g : gslot) drop_graph(g);
+  dev = nullptr;
+  dev_cap = 0;
+  const size_t cap = std::max(bytes * 2, static_cast<size_t>(64) << 10);
+  HIPCHK(hipMalloc(reinterpret_cast<void**>(&dev), cap));
+  dev_cap = cap;
+  return OA_OK;
+}
 
 void Plan::release() {
   for (int k = 0; k < kSlots; ++k) {
@@ -116,6 +136,9 @@ int Plan::submit(hipStream_t stream, bool as_graph) {
     // wait for a step that repeats the previous one
     const bool repeat = (t.rep == -2) && same_launches(t.sig, launches);
     t.sig = launches;
+    // a restart discards what was timed of the abandoned sequence
+    t.acc = 0.0;
+    for (double& m : t.ms) m = 0.0;
     if (!repeat) {
       t.rep = -2;
       return submit_once(stream, as_graph);
@@ -159,18 +182,7 @@ int Plan::submit_once(hipStream_t stream, bool as_graph) {
       HIPCHK(hipEventSynchronize(g.done));
       g.pending = false;
     }
-    if (bytes > dev_cap) {
-      // a larger device arena: every captured graph copies to the old one
-      if (dev) {
-        HIPCHK(hipStreamSynchronize(stream));
-        HIPCHK(hipFree(dev));
-      }
-      dev = nullptr;
-      dev_cap = 0;
-      const size_t cap = std::max(bytes * 2, static_cast<size_t>(64) << 10);
-      HIPCHK(hipMalloc(reinterpret_cast<void**>(&dev), cap));
-      dev_cap = cap;
-    }
+    if (bytes > dev_cap) RC(grow_arena(stream, bytes));
     if (bytes > g.pinned_cap) {
       drop_graph(g);
       if (g.pinned) HIPCHK(hipHostFree(g.pinned));
@@ -222,17 +234,7 @@ int Plan::submit_once(hipStream_t stream, bool as_graph) {
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&pinned[k]), cap, hipHostMallocDefault));
     pinned_cap[k] = cap;
   }
-  if (bytes > dev_cap) {
-    if (dev) {
-      HIPCHK(hipStreamSynchronize(stream));   // earlier launches read the old arena
-      HIPCHK(hipFree(dev));
-    }
-    dev = nullptr;
-    dev_cap = 0;
-    const size_t cap = std::max(bytes * 2, static_cast<size_t>(64) << 10);
-    HIPCHK(hipMalloc(reinterpret_cast<void**>(&dev), cap));
-    dev_cap = cap;
-  }
+  if (bytes > dev_cap) RC(grow_arena(stream, bytes));
   std::memcpy(pinned[k], arena.data(), bytes);
   // stream order: this transfer starts after every earlier launch, which has
   // then finished reading the previous contents of the device arena

@@ -133,6 +133,7 @@ struct Plan {
   // step that may run as a captured graph
   int submit(hipStream_t stream, bool as_graph = false);
   int submit_once(hipStream_t stream, bool as_graph);
+  int grow_arena(hipStream_t stream, size_t bytes);
   // RHS timing over the launches since the last reset
   void reset_timing() { kev_used = 0; }
   int rhs_stats(double* total_ms, long* launches, double* points) const;

@@ -86,6 +86,8 @@ class Server:
         p = oaxis.params_from_text(req.text)
         st = oaxis.Spacetime(p)
         try:
+            if not self.graphs:
+                st.set_graphs(False)      # the elliptic solve kernel by kernel too
             st.initial_data()
             st.initial_aux()
         except Exception:

@@ -86,3 +86,28 @@ def test_elliptic_initial_data_matches_oracle(oaxis_mod, oracle_mod, case):
     print("  worst %s" % (worst,), flush=True)
     bad = {k: v for k, v in errs.items() if not v <= 1e-10}
     assert not bad, bad
+
+
+def test_elliptic_solve_without_graphs_is_identical(oaxis_mod):
+    """With graphs switched off (oa_set_graphs 0) the solve launches every
+    coarse step kernel by kernel: same kernels, same order, so the same
+    iteration counts and bit-identical initial data as the replayed graph."""
+    import numpy as np
+    oaxis = oaxis_mod
+    p = oaxis.params_from_text(CASES["brill_3levels_stationary"][0])
+    out = []
+    for graphs in (True, False):
+        with oaxis.Spacetime(p) as st:
+            if not graphs:
+                st.set_graphs(False)
+            st.initial_data()
+            st.initial_aux()
+            stats = st.elliptic_stats()
+            state = {(lev, name): st.download(name, lev) for lev in range(st.nlevels)
+                     for name in oaxis.state_fields(p)}
+        print("graphs %s: %s" % (graphs, stats), flush=True)
+        out.append((stats, state))
+    (s0, a), (s1, b) = out
+    assert s0["steps0"] == s1["steps0"] and s0["steps1"] == s1["steps1"]
+    for key in a:
+        assert np.array_equal(a[key], b[key]), key
