@@ -221,13 +221,13 @@ struct GlobalView {
 };
 
 struct TilePlane {
-  const double* p;
+  const double* p;      // the tile value of point (ib, jb)
   int ib, jb, tw;
   OA_HD double operator()(int i, int j) const { return p[(i - ib) * tw + (j - jb)]; }
 };
 
 struct TileView {
-  const double* t;
+  const double* t;      // slot 0's value of point (ib, jb)
   int ib, jb, tw, ts;
   int npre;             // tile slot of prefld(0): staged after the evolved fields
   OA_HD TilePlane fld(int s) const { return TilePlane{t + s * ts, ib, jb, tw}; }

@@ -119,6 +119,18 @@ __device__ __forceinline__ T* gmem(T* p) {
   return (T*)g;
 }
 
+// The tile address a thread's stencils are read relative to: its point's
+// value, kHalo rows and columns back (di, dj: the point's position in the
+// staged tile).  The centred stencils then reach offsets 0 .. 6 rows and
+// columns from it, all non-negative, so every read is one LDS instruction
+// with an immediate offset; the empty asm keeps the compiler from splitting
+// the address back into a tile base and a signed per-read offset.
+__device__ __forceinline__ const double* corner_of(const double* tile, int di, int dj) {
+  auto l = (__attribute__((address_space(3))) const double*)(tile + (di - kHalo) * kTileW + (dj - kHalo));
+  asm volatile("" : "+v"(l));
+  return (const double*)l;
+}
+
 // the item of block b in a table launch: the last one whose block0 <= b
 // (block0 ascends; the loads are uniform across the block)
 template <class Item>
@@ -171,7 +183,7 @@ __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, doub
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
   if (bi == nbi - 1 && i < 1 + (nbi - 1) * kTile) return;        // done by the tile before
   if (bj == nbj - 1 && j < G.jlo + (nbj - 1) * kTile) return;
-  const TileView V{tile, ib, jb, kTileW, kTileSize, nf};
+  const TileView V{corner_of(tile, i - ib, j - jb), i - kHalo, j - kHalo, kTileW, kTileSize, nf};
   // phase 1: the geometric auxiliaries of this point, through gaux (own point
   // only, so no synchronisation); the memory barrier keeps the compiler from
   // carrying phase-1 values or loads into phase 2, so the two phases do not
@@ -237,7 +249,7 @@ __global__ void __launch_bounds__(kTile * kTile, 2) k_aux(RhsItem it) {
   const int i = i0 + static_cast<int>(threadIdx.y), j = j0 + static_cast<int>(threadIdx.x);
   if (bi == it.nbi - 1 && i < 1 + (it.nbi - 1) * kTile) return;     // done by the tile before
   if (bj == it.nbj - 1 && j < G.jlo + (it.nbj - 1) * kTile) return;
-  const TileView V{tile, ib, jb, kTileW, kTileSize, NBASE};
+  const TileView V{corner_of(tile, i - ib, j - jb), i - kHalo, j - kHalo, kTileW, kTileSize, NBASE};
   if (interior_tile(G, i0, j0))
     rhs_point<false, false, PHIMODE, true, 1>(G, it.P, it.S, V, nullptr, nullptr, nullptr, it.gaux, i, j);
   else
