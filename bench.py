@@ -286,6 +286,31 @@ def run_bisection(args):
                                                                 "p99_s")} for r in res["rounds"]]}}))
 
 
+def run_bisection_full(args):
+    """The critical-collapse bisection at its stated size: --members (32)
+    concurrent shell pulses (r0 = 5, s = 1) with A spread over [0.01, 0.1],
+    each on 7 levels of 256x512 (dr0 = 0.1, no equatorial symmetry), 1+log +
+    Gamma-driver, RK4, horizon search on the device every 16 coarse steps,
+    --bisection-rounds rounds of --bisection-steps coarse steps each."""
+    from oaxis.bisection import critical_bisection
+    from oaxis import workloads
+    members = args.members if args.members_given else 32
+    res = critical_bisection(lambda a: workloads.bisection_pulse(a), 0.01, 0.1, members=members,
+                             rounds=args.bisection_rounds, steps=args.bisection_steps, ahfind_every=16,
+                             progress=progress)
+    print(json.dumps({"metric": METRIC, "value": res["point_updates"] / res["wall_s"], "unit": "grid-point updates/s",
+                      "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                      "data": "synthetic (shell scalar pulses r0=5, s=1; psi from the elliptic solve)",
+                      "config": {"workload": "critical-collapse bisection: %d amplitudes per round in [0.01, 0.1], "
+                                             "%d round(s) of up to %d coarse steps, 7 levels of 256x512, horizon "
+                                             "search every 16 coarse steps"
+                                             % (members, len(res["rounds"]), args.bisection_steps),
+                                 "bracket": [res["lo"], res["hi"]], "wall_s": res["wall_s"],
+                                 "point_updates": res["point_updates"],
+                                 "rounds": [{k: r[k] for k in ("amplitudes", "horizon", "reasons", "steps", "wall_s",
+                                                                "p50_s", "p99_s")} for r in res["rounds"]]}}))
+
+
 def prepare_spacetimes(texts, threads=8, graphs=True):
     """Spacetimes with their initial data (and its elliptic solve) and the
     initial auxiliary pass, prepared on worker threads: each runs on its own
@@ -459,7 +484,8 @@ def main():
     ap.add_argument("--workload", default="batched_scalar",
                     help="batched_scalar (default), serving, brill, plain_small, plain_256x512, plain_512x1024, "
                          "horizon (one apparent-horizon search only), bisection (reduced critical-collapse "
-                         "bisection; --members per round, --steps rounds)")
+                         "bisection; --members per round, --steps rounds), bisection_full (32 members on 7 "
+                         "levels of 256x512; --bisection-rounds, --bisection-steps)")
     ap.add_argument("--members", type=int, default=64, help="spacetimes of the batched workload")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU time of the baseline sample")
@@ -478,7 +504,11 @@ def main():
                     help="restore the prepared members from checkpoints in DIR, or save them there")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (counter calibration)")
+    ap.add_argument("--bisection-rounds", type=int, default=1, help="rounds of the full-size bisection")
+    ap.add_argument("--bisection-steps", type=int, default=320,
+                    help="coarse steps per evolution of the full-size bisection")
     args = ap.parse_args()
+    args.members_given = any(a == "--members" or a.startswith("--members=") for a in sys.argv[1:])
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -494,6 +524,9 @@ def main():
         return
     if args.workload == "bisection":
         run_bisection(args)
+        return
+    if args.workload == "bisection_full":
+        run_bisection_full(args)
         return
     if args.workload == "horizon":
         # the apparent-horizon search alone (for profiling the finder)

@@ -16,11 +16,13 @@ import numpy as np
 
 
 def critical_bisection(text_of, lo, hi, members=8, rounds=3, steps=200, ahfind_every=16,
-                       memory_budget=200e9, tol=0.0):
+                       memory_budget=200e9, tol=0.0, progress=None, prepare_threads=8):
     """Bisect the amplitude A in [lo, hi]: ``text_of(A)`` gives the par-file
     text of one evolution (it should set mattertype, the grid and the gauge;
     horizon searches every ``ahfind_every`` coarse steps are added here).
-    Returns dict(lo, hi, rounds=[per-round records], wall_s, point_updates)."""
+    progress(text), when given, hears of every ahfind_every coarse steps of a
+    round and of its end.  Returns dict(lo, hi, rounds=[per-round records],
+    wall_s, point_updates)."""
     from .serve import Request, Server
     history = []
     t0 = time.perf_counter()
@@ -29,9 +31,19 @@ def critical_bisection(text_of, lo, hi, members=8, rounds=3, steps=200, ahfind_e
         amps = np.linspace(lo, hi, members)
         reqs = [Request(q, text_of(float(a)) + "\nahfind = .true.\nahfind_every = %d\n" % ahfind_every, steps,
                         0.0, "A=%.10g" % a) for q, a in enumerate(amps)]
-        records, summary = Server(memory_budget, check_every=ahfind_every).run(reqs)
+        step_cb = None
+        if progress is not None:
+            def step_cb(k, n, rnd=rnd):
+                progress("round %d: coarse step %d, %d evolutions running" % (rnd, k, n))
+        # every amplitude of a round starts together and the round advances
+        # them in lockstep
+        server = Server(memory_budget, check_every=ahfind_every, prepare_threads=prepare_threads, gang=True)
+        records, summary = server.run(reqs, step_progress=step_cb, step_every=ahfind_every)
         outcome = [r.reason == "horizon" for r in records]
         total_updates += summary["point_updates"]
+        if progress is not None:
+            progress("round %d done in %.1f s: %d of %d formed a horizon"
+                     % (rnd, summary["wall_s"], sum(r.reason == "horizon" for r in records), len(records)))
         history.append(dict(amplitudes=amps.tolist(), horizon=outcome,
                             steps=[r.steps for r in records], reasons=[r.reason for r in records],
                             masses=[r.horizon.get("mass", 0.0) if r.horizon else 0.0 for r in records],

@@ -66,9 +66,13 @@ class Record:
 
 class Server:
     def __init__(self, memory_budget, max_members=256, check_every=16, alpha_collapse=0.05,
-                 disperse_frac=0.01, on_retire=None, graphs=True, prepare_threads=4):
+                 disperse_frac=0.01, on_retire=None, graphs=True, prepare_threads=4, gang=False):
         import oaxis
         self.oaxis = oaxis
+        # gang: requests that have arrived join together, once all of them are
+        # prepared (a parameter scan advances its members in lockstep instead
+        # of admitting each as soon as its initial data is ready)
+        self.gang = bool(gang)
         self.prepare_threads = max(1, int(prepare_threads))
         self.budget = float(memory_budget)
         self.max_members = int(max_members)
@@ -171,9 +175,11 @@ class Server:
             return "failed"          # a NaN in the state: the request retires with that status
         return None
 
-    def run(self, requests, clock=time.perf_counter, progress=None):
+    def run(self, requests, clock=time.perf_counter, progress=None, step_progress=None, step_every=16):
         """Serve ``requests`` (arrival times relative to the start); returns
-        (records, summary).  progress(record) is called at each retirement."""
+        (records, summary).  progress(record) is called at each retirement,
+        step_progress(coarse steps, active members) every step_every coarse
+        steps of the batch."""
         oaxis = self.oaxis
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active, done = [], []
@@ -187,6 +193,7 @@ class Server:
         pool = ThreadPoolExecutor(max_workers=self.prepare_threads)
         t0 = clock()
         total_updates = 0.0
+        batch_steps = 0
         try:
             while pending or active or preparing:
                 now = clock() - t0
@@ -204,7 +211,7 @@ class Server:
                     preparing.append((req, now, fp, pool.submit(self._prepare, req)))
                     now = clock() - t0
                 # prepared requests join at this step boundary, in admission order
-                while preparing and (preparing[0][3].done() or not active):
+                while preparing and (preparing[0][3].done() or not active or self.gang):
                     req, t_adm, fp, fut = preparing.pop(0)
                     self._admit(req, t_adm, batch, active, used, prepared=fut.result())
                 if not active:
@@ -220,8 +227,11 @@ class Server:
                 if not active:
                     continue
                 batch.coarse_step()
+                batch_steps += 1
                 for a in active:
                     a["steps"] += 1
+                if step_progress is not None and batch_steps % step_every == 0:
+                    step_progress(batch_steps, len(active))
                 retire = self._due_retirements(batch, active)
                 if retire:
                     batch.sync()

@@ -127,9 +127,8 @@ boundtype = radiative
 
 def collapse_pulse(amplitude, Nrtotal=32, Nztotal=32, Nl0=1, dr0=0.125):
     """A central massless scalar pulse phi = A exp(-(rho^2 + z^2)) with the
-    Gamma-driver, equatorial symmetry, RK4.  With psi = 1 on 2 levels of
-    32x32 (dr0 = 0.125) a horizon forms by coarse step 16 for A = 0.6, and
-    none within 80 coarse steps for A = 0.3."""
+    Gamma-driver, equatorial symmetry, RK4, on 2 levels of 32x32 (dr0 =
+    0.125) by default: the reduced bisection's member."""
     return """
 mattertype = scalar
 scalarpotential = none
@@ -147,6 +146,40 @@ drivercsi = 0.75
 integrator = rk4
 order = four
 eqsym = .true.
+Nrtotal = %d
+Nztotal = %d
+Nl0 = %d
+dr0 = %.17g
+dz0 = %.17g
+dtfac = 0.25
+geodiss = 0.1
+scalardiss = 0.1
+boundtype = radiative
+""" % (amplitude, Nrtotal, Nztotal, Nl0, dr0, dr0)
+
+
+def bisection_pulse(amplitude, Nrtotal=256, Nztotal=512, Nl0=6, dr0=0.1):
+    """A member of the critical-collapse bisection: a shell pulse of the real
+    scalar field with r0 = 5, s = 1 and amplitude A, 1+log + Gamma-driver,
+    RK4, on 7 box-in-box levels of 256x512 (rho in (0, 25.6] on the coarsest,
+    no equatorial symmetry)."""
+    return """
+mattertype = scalar
+scalarpotential = none
+idata = scalarpulse
+scalarpulse_type = shell
+scalar_a0 = %.17g
+scalar_r0 = 5.0
+scalar_z0 = 5.0
+scalar_sr0 = 1.0
+scalar_sz0 = 1.0
+slicing = 1+log
+gauge_f = 2.0
+shift = Gammadriver2
+drivercsi = 0.75
+integrator = rk4
+order = four
+eqsym = .false.
 Nrtotal = %d
 Nztotal = %d
 Nl0 = %d
