@@ -19,6 +19,14 @@
 // device at the end of each coarse step; once it fires, every kernel of the
 // later replays returns at once, so the host may run ahead by a whole chunk
 // of steps and reads the outcome after each chunk.
+//
+// ELL_solver = sor selects the reference's other solver, sor_elliptic('flat',
+// 'one') (src/elliptic/sor_elliptic.f90): red-black successive
+// over-relaxation with Chebyshev acceleration, level by level (Robin
+// boundaries on the base level, Dirichlet from the parent's solution on the
+// finer ones), then restriction.  One iteration -- two colour sweeps, their
+// norms, the boundary recurrences, the symmetries and the stopping rule --
+// is captured and replayed the same way.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -291,10 +299,13 @@ struct EllRsArgs {
   const double* uf; const double* vf;
   double* uc; double* vc;
   int ic0, nic, jc0, njc, interporder;
+  int images;            // also write the coarse level's parity images
 };
 
-// restrict(box, level, .false.) for u and v (restrict.f90:88-162) with the
-// coarse level's symmetries (:1136-1149) as images of each written point
+// restrict(box, level, .false.) for u and v (restrict.f90:88-162); with
+// images, the coarse level's symmetries that wavestep applies right after
+// it (:1136-1149) as images of each written point (the SOR solve applies
+// none)
 __global__ void __launch_bounds__(kBlk) k_ell_restrict(EllRsArgs a, const EllCtl* __restrict__ ctl) {
   if (ctl->done) return;
   const long q = static_cast<long>(blockIdx.x) * kBlk + threadIdx.x;
@@ -314,8 +325,12 @@ __global__ void __launch_bounds__(kBlk) k_ell_restrict(EllRsArgs a, const EllCtl
   const double rmax = rmaxf - deltar, zmax = zmaxf - deltaz;
   if (!(r0 > rmin && r0 < rmax && z0 > zmin && z0 < zmax)) return;
   const Cell c = locate(Gf, r0, z0, a.interporder);
-  put(a.uc, Gc, ic, jc, apply(c, a.uf, Gf), 1.0, 1.0);
-  put(a.vc, Gc, ic, jc, apply(c, a.vf, Gf), 1.0, 1.0);
+  if (a.images) {
+    put(a.uc, Gc, ic, jc, apply(c, a.uf, Gf), 1.0, 1.0);
+    put(a.vc, Gc, ic, jc, apply(c, a.vf, Gf), 1.0, 1.0);
+  } else {
+    a.uc[IDX(Gc, ic, jc)] = apply(c, a.uf, Gf);
+  }
 }
 
 // Residual and change of the base level after a coarse step, per block:
@@ -408,6 +423,202 @@ __global__ void __launch_bounds__(kBlk) k_ell_zero(long n, double* __restrict__ 
   if (q < n) f[q] = 0.0;
 }
 
+// ------------------------------------------------------------------ SOR
+// sor (sor_elliptic.f90:535-802) for the flat Laplacian on one level:
+// red-black successive over-relaxation with Chebyshev acceleration.  The
+// reference's arrays run over 1 .. N+ghost; the level's own indices are used
+// here, so its sweep over 2 .. imax-1 is i = 2-g .. Nr-1 (j alike) and its
+// colour (i+j) mod 2 is the same.  The flat coefficients are formed from r
+// at each point (the mixed-derivative ones are zero).
+struct SorCtl {
+  int n;                 // iterations done
+  int done;              // converged or ELL_maxiter reached: later kernels return at once
+  double omega;          // over-relaxation factor of the next pass
+  double anorm;          // running residual norm of the iteration (both passes)
+  double ganorm;         // its value after the second pass
+};
+
+__global__ void k_sor_reset(SorCtl* ctl) {
+  if (threadIdx.x == 0) *ctl = SorCtl{0, 0, 1.0, 0.0, 0.0};
+}
+
+// One colour of one iteration: sor_e = aux + S5 u^4 with u at the start of
+// the iteration (the other colour's pass does not touch this point), the
+// residual, the update; per block the sum of resid^2 (the reference sums in
+// sweep order, here by blocks in a fixed order).
+__global__ void __launch_bounds__(kBlk) k_sor_sweep(KGrid G, EllPlanes E, int color, double* __restrict__ part,
+                                                    const SorCtl* __restrict__ ctl) {
+  __shared__ double ssq[kBlk];
+  const int t = static_cast<int>(threadIdx.x);
+  double sq = 0.0;
+  if (!ctl->done) {
+    const double one = 1.0, half = 0.5, two = 2.0;
+    const double omega = ctl->omega;
+    const int ni = G.Nr - 1 - (2 - G.g) + 1, nj = G.Nz - 1 - (2 - G.g) + 1;
+    const long n = static_cast<long>(ni) * nj;
+    double* __restrict__ u = E.u;
+    for (long q = static_cast<long>(blockIdx.x) * kBlk + t; q < n; q += static_cast<long>(gridDim.x) * kBlk) {
+      const int i = 2 - G.g + static_cast<int>(q / nj), j = 2 - G.g + static_cast<int>(q % nj);
+      if (((i + j) & 1) != color) continue;
+      const long k = IDX(G, i, j);
+      const double r = coord_r(G, i);
+      const double a = (one / G.dr + half / r) / G.dr;
+      const double b = (one / G.dr - half / r) / G.dr;
+      const double c = one / (G.dz * G.dz);
+      const double d = one / (G.dz * G.dz);
+      const double u0 = u[k];
+      const double u2 = u0 * u0;
+      const double e = (-two * (one / (G.dr * G.dr) + one / (G.dz * G.dz)) + E.s1[k]) + E.s5[k] * (u2 * u2);
+      const double resid = a * u[IDX(G, i + 1, j)] + b * u[IDX(G, i - 1, j)] + c * u[IDX(G, i, j + 1)]
+                           + d * u[IDX(G, i, j - 1)] + e * u0 - E.s0[k];
+      sq += resid * resid;
+      u[k] = u0 - omega * resid / e;
+    }
+  }
+  ssq[t] = sq;
+  __syncthreads();
+  for (int h = kBlk / 2; h > 0; h >>= 1) {
+    if (t < h) ssq[t] += ssq[t + h];
+    __syncthreads();
+  }
+  if (t == 0) part[blockIdx.x] = ssq[0];
+}
+
+// after a pass: anorm = sqrt(anorm / (imax jmax)) (the second pass adds its
+// squares to the first pass's norm), then the next omega (:585-600)
+__global__ void k_sor_norm(int nparts, const double* __restrict__ part, double nn, double rjac, int pass,
+                           SorCtl* ctl) {
+  if (threadIdx.x != 0 || ctl->done) return;
+  double sq = 0.0;
+  for (int b = 0; b < nparts; ++b) sq += part[b];
+  SorCtl c = *ctl;
+  if (pass == 0) {
+    c.n += 1;
+    c.anorm = sqrt(sq / nn);
+  } else {
+    c.anorm = sqrt((c.anorm + sq) / nn);
+    c.ganorm = c.anorm;
+  }
+  if (c.n == 1 && pass == 0)
+    c.omega = 1.0 / (1.0 - 0.5 * rjac * rjac);
+  else
+    c.omega = 1.0 / (1.0 - 0.25 * rjac * rjac * c.omega);
+  *ctl = c;
+}
+
+// Robin boundary of the base level (:655-739), uinf = 1: the outer rho
+// boundary, then the upper and (no equatorial symmetry) the lower z
+// boundary, then the corners.  Each line is a recurrence in the order of the
+// reference (the point before is already new, the point after still old),
+// run by one thread over values the block stages in LDS in chunks.
+constexpr int kSorChunk = kBlk;
+constexpr size_t kSorLds = sizeof(double) * (3 * kSorChunk + 1);
+
+struct SorLine {
+  long k0, dk;           // plane offset of the line's first point and the step along it
+  long d1;               // offset from a line point to its interior neighbour (d2 = 2 d1)
+  int n;                 // points on the line
+  double cfix;           // cr (rho boundary) or cz (z boundaries) of the line
+  int sgn;               // +1: outer boundary, -1: lower z boundary
+  bool along_z;          // the line runs along z (the rho boundary)
+  int first;             // index (j or i) of its first point
+};
+
+__device__ void sor_line(const KGrid& G, double* __restrict__ u, const SorLine L, double uinf, double* lds) {
+  double* cur = lds;                     // old values of points q0 .. q0 + chunk
+  double* in1 = lds + kSorChunk + 1;     // the interior neighbours
+  double* in2 = in1 + kSorChunk;
+  const int t = static_cast<int>(threadIdx.x);
+  double prev = 0.0;
+  for (int q0 = 0; q0 < L.n; q0 += kSorChunk) {
+    const int m = min(kSorChunk, L.n - q0);
+    for (int q = t; q <= m; q += kBlk) cur[q] = u[L.k0 + (q0 + q) * L.dk];
+    for (int q = t; q < m; q += kBlk) {
+      const long k = L.k0 + (q0 + q) * L.dk;
+      in1[q] = u[k + L.d1];
+      in2[q] = u[k + 2 * L.d1];
+    }
+    __syncthreads();
+    if (t == 0) {
+      // the point before the line keeps its old value
+      if (q0 == 0) prev = u[L.k0 - L.dk];
+      const double c = L.cfix;
+      for (int q = 0; q < m; ++q) {
+        const int x = L.first + q0 + q;
+        const double w = L.along_z ? coord_z(G, x) / G.dz : coord_r(G, x) / G.dr;
+        const double after = cur[q + 1];
+        double v;
+        if (L.sgn > 0)
+          v = (uinf + 0.5 * c * (4.0 * in1[q] - in2[q]) - 0.5 * w * (after - prev)) / (1.0 + 1.5 * c);
+        else
+          v = (uinf - 0.5 * c * (4.0 * in1[q] - in2[q]) - 0.5 * w * (after - prev)) / (1.0 - 1.5 * c);
+        cur[q] = v;
+        prev = v;
+      }
+    }
+    __syncthreads();
+    for (int q = t; q < m; q += kBlk) u[L.k0 + (q0 + q) * L.dk] = cur[q];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlk) k_sor_robin(KGrid G, EllPlanes E, const SorCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  extern __shared__ double lds[];
+  double* __restrict__ u = E.u;
+  const int g = G.g, Nr = G.Nr, Nz = G.Nz;
+  const double uinf = 1.0;
+  // the rho boundary i = Nr, j = 2-g .. Nz-1
+  sor_line(G, u, SorLine{IDX(G, Nr, 2 - g), 1, -static_cast<long>(G.ld), Nz - 1 - (2 - g) + 1,
+                         coord_r(G, Nr) / G.dr, +1, true, 2 - g}, uinf, lds);
+  // the upper z boundary j = Nz, i = 2-g .. Nr-1
+  sor_line(G, u, SorLine{IDX(G, 2 - g, Nz), G.ld, -1, Nr - 1 - (2 - g) + 1, coord_z(G, Nz) / G.dz, +1, false, 2 - g},
+           uinf, lds);
+  // the lower z boundary j = 1-g, i = 2-g .. Nr-1
+  if (!G.eqsym)
+    sor_line(G, u, SorLine{IDX(G, 2 - g, 1 - g), G.ld, 1, Nr - 1 - (2 - g) + 1, coord_z(G, 1 - g) / G.dz, -1, false,
+                           2 - g}, uinf, lds);
+  if (threadIdx.x == 0) {
+    const double cr = coord_r(G, Nr) / G.dr;
+    double cz = coord_z(G, Nz) / G.dz;
+    u[IDX(G, Nr, Nz)] = (uinf + 0.5 * cr * (4.0 * u[IDX(G, Nr - 1, Nz)] - u[IDX(G, Nr - 2, Nz)])
+                         + 0.5 * cz * (4.0 * u[IDX(G, Nr, Nz - 1)] - u[IDX(G, Nr, Nz - 2)])) / (1.0 + 1.5 * (cr + cz));
+    if (!G.eqsym) {
+      cz = coord_z(G, 1 - g) / G.dz;
+      u[IDX(G, Nr, 1 - g)] = (uinf + 0.5 * cr * (4.0 * u[IDX(G, Nr - 1, 1 - g)] - u[IDX(G, Nr - 2, 1 - g)])
+                              - 0.5 * cz * (4.0 * u[IDX(G, Nr, 2 - g)] - u[IDX(G, Nr, 3 - g)]))
+                             / (1.0 + 1.5 * (cr - cz));
+    }
+  }
+}
+
+// the images on the axis, then (eqsym) the equator (:741-758), and the
+// stopping rule of the iteration (:766-775)
+__global__ void __launch_bounds__(kBlk) k_sor_sym(KGrid G, EllPlanes E, double eps, int maxiter, SorCtl* ctl) {
+  if (ctl->done) return;
+  double* __restrict__ u = E.u;
+  const int g = G.g;
+  const int t = static_cast<int>(threadIdx.x);
+  const int nzt = G.Nz + g, nrt = G.Nr + g;
+  for (int q = t; q < g * nzt; q += kBlk) {
+    const int i = 1 + q / nzt, j = 1 - g + q % nzt;
+    u[IDX(G, 1 - i, j)] = u[IDX(G, i, j)];
+  }
+  __syncthreads();
+  if (G.eqsym) {
+    for (int q = t; q < g * nrt; q += kBlk) {
+      const int j = 1 + q / nrt, i = 1 - g + q % nrt;
+      u[IDX(G, i, 1 - j)] = u[IDX(G, i, j)];
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    SorCtl c = *ctl;
+    if (c.ganorm < eps || c.n >= maxiter) c.done = 1;
+    *ctl = c;
+  }
+}
+
 int blocks(long n) { return static_cast<int>((n + kBlk - 1) / kBlk); }
 
 int launched(const char* what) {
@@ -445,6 +656,8 @@ struct EllSolver {
   double* part = nullptr;
   int nparts = 0;
   EllCtl* host_ctl = nullptr;    // pinned
+  SorCtl* sctl = nullptr;        // the SOR solve's control block
+  SorCtl* host_sctl = nullptr;   // pinned
   double dt0 = 0.0;
   int stationary = 1;
 
@@ -452,6 +665,7 @@ struct EllSolver {
     if (st) (void)hipStreamSynchronize(st);
     if (arena) (void)hipFree(arena);
     if (host_ctl) (void)hipHostFree(host_ctl);
+    if (host_sctl) (void)hipHostFree(host_sctl);
   }
 
   int setup(oa_handle* hh, int stationary_exit) {
@@ -474,12 +688,15 @@ struct EllSolver {
     nparts = 256;
     const size_t ctl_doubles = (sizeof(EllCtl) + sizeof(double) - 1) / sizeof(double);
     const size_t clk_doubles = (sizeof(EllClock) * nlev + sizeof(double) - 1) / sizeof(double);
-    const size_t bytes = sizeof(double) * (total + 3 * nparts + ctl_doubles + clk_doubles);
+    const size_t sctl_doubles = (sizeof(SorCtl) + sizeof(double) - 1) / sizeof(double);
+    const size_t bytes = sizeof(double) * (total + 3 * nparts + ctl_doubles + clk_doubles + sctl_doubles);
     ELLHIP(hipMalloc(&arena, bytes), "device allocation");
     part = arena + total;
     ctl = reinterpret_cast<EllCtl*>(part + 3 * nparts);
     clocks = reinterpret_cast<EllClock*>(part + 3 * nparts + ctl_doubles);
+    sctl = reinterpret_cast<SorCtl*>(part + 3 * nparts + ctl_doubles + clk_doubles);
     ELLHIP(hipHostMalloc(reinterpret_cast<void**>(&host_ctl), sizeof(EllCtl)), "pinned allocation");
+    ELLHIP(hipHostMalloc(reinterpret_cast<void**>(&host_sctl), sizeof(SorCtl)), "pinned allocation");
     lev.resize(nlev);
     for (int l = 0; l < nlev; ++l) {
       EllLevelArgs& a = lev[l];
@@ -603,6 +820,7 @@ struct EllSolver {
       rs.uc = a.P.u;
       rs.vc = a.P.v;
       rs.interporder = p.interporder;
+      rs.images = 1;
       restrict_box(f.G, G, &rs.ic0, &rs.nic, &rs.jc0, &rs.njc);
       hipLaunchKernelGGL(k_ell_restrict, dim3(blocks(static_cast<long>(rs.nic) * rs.njc)), dim3(kBlk), 0, st, rs,
                          ctl);
@@ -690,8 +908,142 @@ struct EllSolver {
     return OA_OK;
   }
 
+  // ------------------------------------------------------------ SOR
+  // one iteration of sor() on a level (sor_elliptic.f90:575-775) as launches
+  int sor_iteration(int level, bool robin, double rjac) {
+    const EllLevelArgs& a = lev[level];
+    const KGrid& G = a.G;
+    const double nn = static_cast<double>(G.Nr + G.g) * static_cast<double>(G.Nz + G.g);
+    for (int color = 0; color < 2; ++color) {
+      hipLaunchKernelGGL(k_sor_sweep, dim3(nparts), dim3(kBlk), 0, st, G, a.P, color, part, sctl);
+      ELLRC(launched("SOR sweep"));
+      hipLaunchKernelGGL(k_sor_norm, dim3(1), dim3(64), 0, st, nparts, part, nn, rjac, color, sctl);
+      ELLRC(launched("SOR norm"));
+    }
+    if (robin) {
+      hipLaunchKernelGGL(k_sor_robin, dim3(1), dim3(kBlk), kSorLds, st, G, a.P, sctl);
+      ELLRC(launched("SOR Robin boundary"));
+    }
+    hipLaunchKernelGGL(k_sor_sym, dim3(1), dim3(kBlk), 0, st, G, a.P, h->p.ELL_epsilon, h->p.ELL_maxiter, sctl);
+    return launched("SOR symmetries");
+  }
+
+  // sor() on one level until its residual falls below ELL_epsilon or
+  // ELL_maxiter iterations; the iteration is captured once as a graph and
+  // replayed in chunks, unless graphs are switched off
+  int sor_level(int level, bool robin, double rjac, int* iters, double* res) {
+    hipLaunchKernelGGL(k_sor_reset, dim3(1), dim3(64), 0, st, sctl);
+    ELLRC(launched("SOR reset"));
+    const bool use_graph = h->plan.graphs || h->plan.tuner.enabled;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+      ELLHIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+      const int rc = sor_iteration(level, robin, rjac);
+      hipGraph_t captured = nullptr;
+      const hipError_t ec = hipStreamEndCapture(st, &captured);
+      if (rc != OA_OK) {
+        if (captured) (void)hipGraphDestroy(captured);
+        return rc;
+      }
+      ELLHIP(ec, "graph capture");
+      graph = captured;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        ELLHIP(ei, "graph instantiation");
+      }
+    }
+    int status = OA_OK;
+    long launched_its = 0;
+    for (;;) {
+      const int chunk = (launched_its == 0) ? 100 : 64;
+      for (int k = 0; k < chunk && status == OA_OK; ++k) {
+        if (!use_graph) {
+          status = sor_iteration(level, robin, rjac);
+          continue;
+        }
+        const hipError_t e = hipGraphLaunch(exec, st);
+        if (e != hipSuccess) {
+          oa_set_error(std::string("elliptic solve: graph launch: ") + hipGetErrorString(e));
+          status = OA_ERR_HIP;
+        }
+      }
+      launched_its += chunk;
+      if (status != OA_OK) break;
+      const hipError_t e1 = hipMemcpyAsync(host_sctl, sctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st);
+      const hipError_t e2 = hipStreamSynchronize(st);
+      if (e1 != hipSuccess || e2 != hipSuccess) {
+        oa_set_error("elliptic solve: reading the SOR state failed");
+        status = OA_ERR_HIP;
+        break;
+      }
+      if (host_sctl->done) break;
+      if (launched_its > static_cast<long>(h->p.ELL_maxiter) + 200) {
+        oa_set_error("elliptic solve: SOR did not stop within ELL_maxiter iterations");
+        status = OA_ERR_STATE;
+        break;
+      }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (status != OA_OK) return status;
+    *iters = host_sctl->n;
+    *res = host_sctl->ganorm;
+    return OA_OK;
+  }
+
+  // sor_elliptic('flat', 'one') (sor_elliptic.f90:2-530): u = 1 on every
+  // level (the coefficients pass set it), the base level with Robin
+  // boundaries, each finer level from the interpolation of its parent's
+  // solution (initial guess and Dirichlet boundary), then every level
+  // restricted onto its parent, level 1 first, without images
+  int sor_solve(int* steps0, int* steps1, double* gres) {
+    const KGrid& G0 = lev[0].G;
+    const double smallpi = acos(-1.0);
+    const double rjac = (G0.dz / G0.dr * cos(smallpi / static_cast<double>(G0.Nr + G0.g))
+                         + G0.dr / G0.dz * cos(smallpi / static_cast<double>(G0.Nz + G0.g)))
+                        / (G0.dz / G0.dr + G0.dr / G0.dz);
+    ELLRC(sor_level(0, true, rjac, steps0, gres));
+    *steps1 = 0;
+    for (int l = 1; l < nlev; ++l) {
+      const EllLevelArgs& f = lev[l];
+      const EllLevelArgs& c = lev[l - 1];
+      const long n = static_cast<long>(f.G.Nr + f.G.g) * (f.G.Nz + f.G.g);
+      hipLaunchKernelGGL(k_ell_inject, dim3(blocks(n)), dim3(kBlk), 0, st, f.G, c.G, c.P.u, f.P.u, f.P.v,
+                         h->p.interporder);
+      ELLRC(launched("injection"));
+      int it = 0;
+      double r = 0.0;
+      ELLRC(sor_level(l, false, rjac, &it, &r));
+      *steps1 += it;
+    }
+    // k_ell_restrict runs while the wave solve's control block says not done
+    hipLaunchKernelGGL(k_ell_reset, dim3(1), dim3(64), 0, st, ctl, 0, clocks, nlev);
+    ELLRC(launched("reset"));
+    for (int l = 1; l < nlev; ++l) {
+      const EllLevelArgs& f = lev[l];
+      const EllLevelArgs& c = lev[l - 1];
+      EllRsArgs rs{};
+      rs.Gf = f.G;
+      rs.Gc = c.G;
+      rs.uf = f.P.u;
+      rs.vf = f.P.v;
+      rs.uc = c.P.u;
+      rs.vc = c.P.v;
+      rs.interporder = h->p.interporder;
+      rs.images = 0;
+      restrict_box(f.G, c.G, &rs.ic0, &rs.nic, &rs.jc0, &rs.njc);
+      hipLaunchKernelGGL(k_ell_restrict, dim3(blocks(static_cast<long>(rs.nic) * rs.njc)), dim3(kBlk), 0, st, rs,
+                         ctl);
+      ELLRC(launched("restrict"));
+    }
+    return OA_OK;
+  }
+
   // wave_elliptic: the base level alone, then (refinement) all levels
   int solve(int* steps0, int* steps1, double* gres) {
+    if (h->p.ELL_solver == OA_ELL_SOR) return sor_solve(steps0, steps1, gres);
     const int nlmax_old = nlev - 1;
     *steps1 = 0;
     ELLRC(phase(nlmax_old > 0 ? 0 : nlmax_old, 0, steps0, gres));

@@ -1,8 +1,9 @@
 /*
  * ORACLE -- TEST INFRASTRUCTURE ONLY (see or_internal.h).
  *
- * The WaveElliptic solver of the Hamiltonian constraint used by Brill-wave
- * and scalar-pulse initial data: wave_elliptic('flat', 'one')
+ * The elliptic solvers of the Hamiltonian constraint used by Brill-wave
+ * and scalar-pulse initial data (ELL_solver = wave or sor).  WaveElliptic:
+ * wave_elliptic('flat', 'one')
  * (src/elliptic/wave_elliptic.f90:2-532) and its recursive step wavestep
  * (:542-1164), for box 0.  The elliptic equation
  *     lap u + S1 u + S5 u^5 = S0      (flat Laplacian in (rho, z))
@@ -404,6 +405,139 @@ static void wave_elliptic(or_handle* H, int* steps, double* res) {
   free(ssave);
 }
 
+/* ------------------------------------------------------------------ SOR
+ * sor(a, .., j, u, imax, jmax, rjac, uinf, bound) (sor_elliptic.f90:535-802)
+ * for the flat Laplacian, on one level: red-black successive
+ * over-relaxation with Chebyshev acceleration (the mixed-derivative
+ * coefficients of the flat case are zero, so their products add exact
+ * zeros and are left out).  The reference's arrays run over 1 .. N+ghost;
+ * here the level's own indices i = 1-g .. Nr, j = 1-g .. Nz are used, so
+ * its sweep over 2 .. imax-1 is i = 2-g .. Nr-1 (j alike), and its colour
+ * (i+j) mod 2 is the same.  robin: the outer and (without equatorial
+ * symmetry) lower boundaries follow the Robin condition with uinf = 1 (base
+ * level); otherwise every point outside the sweep keeps its value
+ * (Dirichlet, fine levels).  Returns the iterations taken (ELL_maxiter when
+ * the residual never fell below ELL_epsilon). */
+static int sor_level(const oa_params* p, const or_level* L, or_ell_level* E, double rjac, int robin, double* res) {
+  const int g = L->g, Nr = L->Nr, Nz = L->Nz;
+  const double one = 1.0, half = 0.5, two = 2.0, uinf = 1.0;
+  const double dr = L->dr, dz = L->dz;
+  double* u = E->u;
+  const double nn = (double)(Nr + g) * (double)(Nz + g);
+  double omega = 1.0, ganorm = 0.0;
+  for (int n = 1; n <= p->ELL_maxiter; ++n) {
+    double anorm = 0.0;
+    /* sor_e = sor_aux + S5 u^4 at the start of the iteration (:606-611) */
+    for (int ipass = 1; ipass <= 2; ++ipass) {
+      for (int i = 2 - g; i <= Nr - 1; ++i)
+        for (int j = 2 - g; j <= Nz - 1; ++j) {
+          if (((i + j) & 1) != (ipass - 1)) continue;
+          const int k = IX(L, i, j);
+          const double r = L->r[k];
+          const double a = (one / dr + half / r) / dr;
+          const double b = (one / dr - half / r) / dr;
+          const double c = one / (dz * dz);
+          const double d = one / (dz * dz);
+          const double u2 = u[k] * u[k];
+          const double e = (-two * (one / (dr * dr) + one / (dz * dz)) + E->S1[k]) + E->S5[k] * (u2 * u2);
+          const double resid = a * u[IX(L, i + 1, j)] + b * u[IX(L, i - 1, j)] + c * u[IX(L, i, j + 1)]
+                               + d * u[IX(L, i, j - 1)] + e * u[k] - E->S0[k];
+          anorm = anorm + resid * resid;
+          u[k] = u[k] - omega * resid / e;
+        }
+      anorm = sqrt(anorm / nn);
+      if (n == 1 && ipass == 1)
+        omega = 1.0 / (1.0 - 0.5 * rjac * rjac);
+      else
+        omega = 1.0 / (1.0 - 0.25 * rjac * rjac * omega);
+    }
+    ganorm = anorm;
+    if (robin) {
+      /* outer rho boundary, then the upper and (no equatorial symmetry) the
+       * lower z boundary, then the corners (:655-739) */
+      double cr = L->r[IX(L, Nr, 1)] / dr, cz;
+      for (int j = 2 - g; j <= Nz - 1; ++j) {
+        cz = L->z[IX(L, 1, j)] / dz;
+        u[IX(L, Nr, j)] = (uinf + 0.5 * cr * (4.0 * u[IX(L, Nr - 1, j)] - u[IX(L, Nr - 2, j)])
+                           - 0.5 * cz * (u[IX(L, Nr, j + 1)] - u[IX(L, Nr, j - 1)])) / (1.0 + 1.5 * cr);
+      }
+      cz = L->z[IX(L, 1, Nz)] / dz;
+      for (int i = 2 - g; i <= Nr - 1; ++i) {
+        cr = L->r[IX(L, i, 1)] / dr;
+        u[IX(L, i, Nz)] = (uinf + 0.5 * cz * (4.0 * u[IX(L, i, Nz - 1)] - u[IX(L, i, Nz - 2)])
+                           - 0.5 * cr * (u[IX(L, i + 1, Nz)] - u[IX(L, i - 1, Nz)])) / (1.0 + 1.5 * cz);
+      }
+      if (!p->eqsym) {
+        cz = L->z[IX(L, 1, 1 - g)] / dz;
+        for (int i = 2 - g; i <= Nr - 1; ++i) {
+          cr = L->r[IX(L, i, 1)] / dr;
+          u[IX(L, i, 1 - g)] = (uinf - 0.5 * cz * (4.0 * u[IX(L, i, 2 - g)] - u[IX(L, i, 3 - g)])
+                                - 0.5 * cr * (u[IX(L, i + 1, 1 - g)] - u[IX(L, i - 1, 1 - g)])) / (1.0 - 1.5 * cz);
+        }
+      }
+      cr = L->r[IX(L, Nr, Nz)] / dr;
+      cz = L->z[IX(L, Nr, Nz)] / dz;
+      u[IX(L, Nr, Nz)] = (uinf + 0.5 * cr * (4.0 * u[IX(L, Nr - 1, Nz)] - u[IX(L, Nr - 2, Nz)])
+                          + 0.5 * cz * (4.0 * u[IX(L, Nr, Nz - 1)] - u[IX(L, Nr, Nz - 2)])) / (1.0 + 1.5 * (cr + cz));
+      if (!p->eqsym) {
+        cr = L->r[IX(L, Nr, 1 - g)] / dr;
+        cz = L->z[IX(L, Nr, 1 - g)] / dz;
+        u[IX(L, Nr, 1 - g)] = (uinf + 0.5 * cr * (4.0 * u[IX(L, Nr - 1, 1 - g)] - u[IX(L, Nr - 2, 1 - g)])
+                               - 0.5 * cz * (4.0 * u[IX(L, Nr, 2 - g)] - u[IX(L, Nr, 3 - g)]))
+                              / (1.0 + 1.5 * (cr - cz));
+      }
+    }
+    ell_symmetries(p, L, u);
+    if (ganorm < p->ELL_epsilon) {
+      *res = ganorm;
+      return n;
+    }
+  }
+  *res = ganorm;
+  return p->ELL_maxiter;
+}
+
+/* sor_elliptic('flat', 'one') (sor_elliptic.f90:2-530): u = 1, the base
+ * level with Robin boundaries, then each finer level in turn from the
+ * interpolation of its parent's solution (initial guess and Dirichlet
+ * boundary), then restriction of every level onto its parent, level 1 first
+ * (restrict.f90 with all = .false.; no symmetries are applied after it).
+ * steps[0]: iterations on the base level, steps[1]: their sum over the finer
+ * levels. */
+static void sor_elliptic(or_handle* H, int* steps, double* res) {
+  const oa_params* p = &H->p;
+  for (int l = 0; l < H->nlev; ++l) {
+    or_ell_level* E = &H->ell[l];
+    ALL((&H->L[l]), k) E->u[k] = 1.0;
+  }
+  /* Jacobi spectral radius of the base level's flat Laplacian (:140-143) */
+  const or_level* L0 = &H->L[0];
+  const double smallpi = acos(-1.0);
+  const double rjac = (L0->dz / L0->dr * cos(smallpi / (double)(L0->Nr + L0->g))
+                       + L0->dr / L0->dz * cos(smallpi / (double)(L0->Nz + L0->g)))
+                      / (L0->dz / L0->dr + L0->dr / L0->dz);
+  steps[0] = sor_level(p, L0, &H->ell[0], rjac, 1, res);
+  steps[1] = 0;
+  for (int l = 1; l < H->nlev; ++l) {
+    const or_level* L = &H->L[l];
+    const or_level* Lc = &H->L[l - 1];
+    double* u = H->ell[l].u;
+    const int g = L->g;
+    for (int m = 0; m <= L->Nr + g - 1; ++m)
+      for (int n = 0; n <= L->Nz + g - 1; ++n) {
+        const double r0 = L->rmin + (double)m * L->dr;
+        const double z0 = L->zmin + (double)n * L->dz;
+        const int i = (int)lround((r0 - L->rmin) / L->dr) + 1 - g;
+        const int j = (int)lround((z0 - L->zmin) / L->dz) + 1 - g;
+        const double aux2 = or_interp(p, Lc, H->ell[l - 1].u, r0, z0);
+        if (i >= 1 - g && i <= L->Nr && j >= 1 - g && j <= L->Nz) u[IX(L, i, j)] = aux2;
+      }
+    double lres = 0.0;
+    steps[1] += sor_level(p, L, &H->ell[l], rjac, 0, &lres);
+  }
+  for (int l = 1; l < H->nlev; ++l) ell_restrict(p, &H->L[l], &H->L[l - 1], H->ell[l].u, H->ell[l - 1].u);
+}
+
 /* Coefficients of the Hamiltonian-constraint equation and its solution:
  * Brill waves (idata_brillwave.f90:88-105, :128-135) and the scalar pulse
  * (idata_scalarpulse.f90:143-178, time-symmetric, smallpi = acos(-1d0)).
@@ -436,7 +570,10 @@ int or_solve_hamiltonian(or_handle* H) {
       }
     }
   }
-  wave_elliptic(H, H->ell_steps, &H->ell_res);
+  if (p->ELL_solver == OA_ELL_SOR)
+    sor_elliptic(H, H->ell_steps, &H->ell_res);
+  else
+    wave_elliptic(H, H->ell_steps, &H->ell_res);
   for (int l = 0; l < H->nlev; ++l) {
     or_level* L = &H->L[l];
     memcpy(L->psi, H->ell[l].u, sizeof(double) * (size_t)L->n);

@@ -148,7 +148,8 @@ def test_supported_configurations():
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = bosonstar\n", "idata"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = rotboson\n", "idata"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = BrillWave\nELL_solver = none\n", "elliptic solver"),
-    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = BrillWave\nELL_solver = sor\n", "sor"),
+    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = BrillWave\nELL_solver = sor\nELL_maxiter = 0\n",
+     "ELL_maxiter"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nmattertype = scalar\nidata = scalarpulse\nELL_solver = none\n",
      "elliptic solver"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = scalarpulse\n", "mattertype"),
