@@ -179,7 +179,20 @@ __global__ void __launch_bounds__(256) k_ah_fields(KGrid G, const double* __rest
 // cell inside the level (one cell inside level 0's edges); the stencil rows
 // and columns are clamped to the array, which matters only where a weight
 // is exactly zero.
-__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N], double* sv) {
+//
+// The 4 x 4 stencil records of the last cell each lane read stay in its
+// registers (AhCell): the adaptive RK4 of a shot takes many small steps, so
+// consecutive evaluations mostly fall in the same cell of the same level and
+// reuse them instead of waiting for another round of global loads.  The
+// values, and so the results, are the same either way.
+struct AhCell {
+  const double* q;       // level records the cell belongs to (null: empty)
+  int i0, j0;            // its lower-left stencil point + (1, 1)
+  double fa[4][4];       // quantity f of the records (i0-1+a, j0-1+b), indices clamped
+};
+
+__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N], double* sv,
+                          AhCell& cell) {
   const KGrid& G = L.G;
   const int g = G.g;
   const double r1g = (static_cast<double>(1 - g) - 0.5) * G.dr;
@@ -196,17 +209,29 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
   double* slot = sv + (threadIdx.x >> 5) * AF_N;
   // quantity f of one point record
   auto at = [&](int i, int j) { return L.q[IDX(G, ci(i), cj(j)) * AF_N + f]; };
+  if (cell.q != L.q || cell.i0 != i0 || cell.j0 != j0) {
+    if (f < AF_N) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cell.fa[i][j] = at(i0 - 1 + i, j0 - 1 + j);
+    }
+    cell.q = L.q;
+    cell.i0 = i0;
+    cell.j0 = j0;
+  }
+  const auto& fa = cell.fa;
   double out = 0.0;
   if (deltar == 0.0 && deltaz == 0.0) {
-    if (f < AF_N) out = at(i0, j0);
+    if (f < AF_N) out = fa[1][1];
   } else {
     const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
                      ((z0 - L.zmin) < G.dz);
     const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
     if (order == OA_INTERP_BILINEAR || lin) {
       if (f < AF_N)
-        out = at(i0, j0) * (1.0 - auxr) * (1.0 - auxz) + at(i0 + 1, j0) * auxr * (1.0 - auxz) +
-              at(i0, j0 + 1) * (1.0 - auxr) * auxz + at(i0 + 1, j0 + 1) * auxr * auxz;
+        out = fa[1][1] * (1.0 - auxr) * (1.0 - auxz) + fa[2][1] * auxr * (1.0 - auxz) +
+              fa[1][2] * (1.0 - auxr) * auxz + fa[2][2] * auxr * auxz;
     } else {
       double w[4][4];
 #pragma unroll
@@ -229,11 +254,6 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
         }
       }
       if (f < AF_N) {
-        double fa[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) fa[i][j] = at(i0 - 1 + i, j0 - 1 + j);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -252,7 +272,7 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
 // h'' (sdh) and the area element (sarea) at (h, h', theta) for a horizon
 // centred at zcenter (:1983-2752).  Returns the error flag.
 __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, double zcenter, double& sdh,
-                         double& sarea, double* sv) {
+                         double& sarea, double* sv, AhCell& cell) {
   const AhLevel& L0 = c.L[0];
   const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
   const double smallpi = M_PI;
@@ -284,7 +304,7 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
     if (r0 > L.rmin + L.G.dr && r0 < L.rmax - L.G.dr && z0 > L.zmin + L.G.dz && z0 < L.zmax - L.G.dz) lev = ll;
   }
   double v[AF_N];
-  ah_interp(c.L[lev], c.interporder, r0, z0, v, sv);
+  ah_interp(c.L[lev], c.interporder, r0, z0, v, sv, cell);
   if (err) return 1;
 
   const double interp_psi4 = (v[AF_psi] * v[AF_psi]) * (v[AF_psi] * v[AF_psi]);
@@ -451,6 +471,9 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
   const double sixth = 1.0 / 6.0;
   int i = 0, err = 0;
   double sdh = 0.0, sarea = 0.0;
+  AhCell cell;
+  cell.q = nullptr;
+  cell.i0 = cell.j0 = 0;
   for (int steps = 0; std::fabs(theta - thetafinal) > 1.0e-3; ++steps) {
     if (steps >= kAhMaxSteps) {
       err = 1;
@@ -464,7 +487,7 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
     const double dth2 = half * dth;
     double rad1, drad1, area1, drad_aux;
     double k_1, k_2, k_3, k_4, l_1, l_2, l_3, l_4, m_1, m_2, m_3, m_4;
-    if ((err = ah_source(c, rad, drad, theta, zcenter, sdh, sarea, sv))) break;
+    if ((err = ah_source(c, rad, drad, theta, zcenter, sdh, sarea, sv, cell))) break;
     k_1 = dth * drad;
     l_1 = dth * sdh;
     m_1 = std::fabs(dth2) * sarea;
@@ -479,15 +502,15 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
       const double h = big ? dth : dth2;
       const double k1 = big ? k_1 : half * k_1;
       const double l1 = big ? l_1 : half * l_1;
-      int e = ah_source(c, rad + 0.5 * k1, drad + 0.5 * l1, theta + 0.5 * h, zcenter, sdh, sarea, sv);
+      int e = ah_source(c, rad + 0.5 * k1, drad + 0.5 * l1, theta + 0.5 * h, zcenter, sdh, sarea, sv, cell);
       k_2 = h * (drad + 0.5 * l1);
       l_2 = h * sdh;
       m_2 = std::fabs(dth2) * sarea;
-      e |= ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * h, zcenter, sdh, sarea, sv);
+      e |= ah_source(c, rad + 0.5 * k_2, drad + 0.5 * l_2, theta + 0.5 * h, zcenter, sdh, sarea, sv, cell);
       k_3 = h * (drad + 0.5 * l_2);
       l_3 = h * sdh;
       m_3 = std::fabs(dth2) * sarea;
-      e |= ah_source(c, rad + k_3, drad + l_3, theta + h, zcenter, sdh, sarea, sv);
+      e |= ah_source(c, rad + k_3, drad + l_3, theta + h, zcenter, sdh, sarea, sv, cell);
       k_4 = h * (drad + l_3);
       l_4 = h * sdh;
       m_4 = std::fabs(dth2) * sarea;
@@ -513,19 +536,19 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
     }
     if (err) break;
 
-    if ((err = ah_source(c, rad1, drad1, theta + dth, zcenter, sdh, sarea, sv))) break;
+    if ((err = ah_source(c, rad1, drad1, theta + dth, zcenter, sdh, sarea, sv, cell))) break;
     k_1 = dth2 * drad1;
     l_1 = dth2 * sdh;
     m_1 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad1 + 0.5 * k_1, drad1 + 0.5 * l_1, theta + 1.5 * dth2, zcenter, sdh, sarea, sv))) break;
+    if ((err = ah_source(c, rad1 + 0.5 * k_1, drad1 + 0.5 * l_1, theta + 1.5 * dth2, zcenter, sdh, sarea, sv, cell))) break;
     k_2 = dth2 * (drad1 + 0.5 * l_1);
     l_2 = dth2 * sdh;
     m_2 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad1 + 0.5 * k_2, drad1 + 0.5 * l_2, theta + 1.5 * dth2, zcenter, sdh, sarea, sv))) break;
+    if ((err = ah_source(c, rad1 + 0.5 * k_2, drad1 + 0.5 * l_2, theta + 1.5 * dth2, zcenter, sdh, sarea, sv, cell))) break;
     k_3 = dth2 * (drad1 + 0.5 * l_2);
     l_3 = dth2 * sdh;
     m_3 = std::fabs(dth2) * sarea;
-    if ((err = ah_source(c, rad1 + k_3, drad1 + l_3, theta + dth, zcenter, sdh, sarea, sv))) break;
+    if ((err = ah_source(c, rad1 + k_3, drad1 + l_3, theta + dth, zcenter, sdh, sarea, sv, cell))) break;
     k_4 = dth2 * (drad1 + l_3);
     l_4 = dth2 * sdh;
     m_4 = std::fabs(dth2) * sarea;
@@ -585,7 +608,10 @@ __global__ void __launch_bounds__(kShotBlock) k_ah_dzphi(const AhCtx* __restrict
                                                          double z0, double* __restrict__ out) {
   __shared__ double sv[2 * AF_N];
   double v[AF_N];
-  ah_interp(c->L[level], c->interporder, r0, z0, v, sv);
+  AhCell cell;
+  cell.q = nullptr;
+  cell.i0 = cell.j0 = 0;
+  ah_interp(c->L[level], c->interporder, r0, z0, v, sv, cell);
   if (threadIdx.x == 0) out[0] = v[AF_Dzphi];
 }
 
