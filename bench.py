@@ -308,7 +308,10 @@ def run_bisection_full(args):
                                  "bracket": [res["lo"], res["hi"]], "wall_s": res["wall_s"],
                                  "point_updates": res["point_updates"],
                                  "rounds": [{k: r[k] for k in ("amplitudes", "horizon", "reasons", "steps", "wall_s",
-                                                                "p50_s", "p99_s")} for r in res["rounds"]]}}))
+                                                                "first_step_s", "p50_s", "p99_s")}
+                                            for r in res["rounds"]],
+                                 "evolution_updates_per_s": res["point_updates"] / sum(
+                                     r["wall_s"] - (r["first_step_s"] or 0.0) for r in res["rounds"])}}))
 
 
 def prepare_spacetimes(texts, threads=8, graphs=True):

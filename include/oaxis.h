@@ -343,6 +343,20 @@ int oa_ahfind(oa_handle* h, int k, oa_horizon* out);
  * quantities the search interpolated, so it needs no state; a later search
  * of the same spacetime replaces them (OA_ERR_STATE: search again). */
 int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n);
+/* oa_ahfind in two calls, so that a search can run while the spacetime
+ * evolves on: oa_ahfind_begin forms the quantities the search interpolates
+ * from the current state (in order with the spacetime's work, so a later
+ * step cannot change what it reads) and returns a search;
+ * oa_ahfind_end runs its shots on a stream of their own, concurrently with
+ * whatever the caller enqueued in between (a coarse step, typically), and
+ * returns the result oa_ahfind would have returned at the begin.  No
+ * counterpart in the reference, which searches and then evolves; for the
+ * serving loop, whose searches would otherwise leave the GPU idle between
+ * steps.  A spacetime with a begun search may not begin another, be
+ * destroyed or leave its batch before the search ends (OA_ERR_STATE). */
+typedef struct oa_ahsearch oa_ahsearch;
+int oa_ahfind_begin(oa_handle* h, int k, oa_ahsearch** search);
+int oa_ahfind_end(oa_ahsearch* search, oa_horizon* out);
 
 /* Device memory (bytes) oa_create allocates for these parameters: what a
  * scheduler admitting spacetimes by HBM footprint needs. */
@@ -385,6 +399,9 @@ int oa_batch_launch_stats(const oa_batch* b, long* launches, long* items);
  * one launch.  For the serving loop, which checks many spacetimes for
  * horizon formation at once. */
 int oa_batch_ahfind(oa_batch* b, int n, oa_handle* const* members, const int* k, oa_horizon* out);
+/* oa_batch_ahfind in two calls, as oa_ahfind_begin / oa_ahfind_end (n >= 1;
+ * out of oa_ahfind_end then holds n results, in the order of members). */
+int oa_batch_ahfind_begin(oa_batch* b, int n, oa_handle* const* members, const int* k, oa_ahsearch** search);
 /* HIP-event timing of every RHS launch during oa_batch_evolve (off by
  * default): total kernel time (ms), launches, and grid points (Nr x Nz of
  * each item) those launches advanced by one substep. */

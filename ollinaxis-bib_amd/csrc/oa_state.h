@@ -91,6 +91,7 @@ void finish_level_step(oa_handle* h, int level);   // advance the level clock, c
 // appended to `plan` (oa_plan.cpp)
 int plan_level(Plan& plan, const std::vector<oa_handle*>& all, int level);
 void ah_release(oa_handle* h);                       // free the horizon finder's device memory
+int ah_pending(const oa_handle* h);                  // begun horizon searches not yet ended
 int ah_record_doubles();                             // horizon finder: doubles per grid point
 double ah_fixed_bytes(int n_ah);                     // horizon finder: per-handle device bytes besides those
 

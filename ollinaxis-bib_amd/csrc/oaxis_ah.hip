@@ -668,6 +668,9 @@ struct AhState {
   double last_z[3] = {0.0, 0.0, 0.0}, last_zc[3] = {0.0, 0.0, 0.0};
   int n_of[3] = {0, 0, 0};
   double* d_scalar = nullptr;             // one value back from the device
+  hipStream_t search_stream = nullptr;    // the shots of a begun search (oa_*ahfind_begin)
+  hipEvent_t fields_done = nullptr;       // the point records of a begun search are formed
+  int pending = 0;                        // a begun search awaits its end
 };
 
 namespace {
@@ -915,9 +918,13 @@ void ah_release(oa_handle* h) {
   if (a->d_out) (void)hipFree(a->d_out);
   if (a->h_in) (void)hipHostFree(a->h_in);
   if (a->h_out) (void)hipHostFree(a->h_out);
+  if (a->search_stream) (void)hipStreamDestroy(a->search_stream);
+  if (a->fields_done) (void)hipEventDestroy(a->fields_done);
   delete a;
   h->ah = nullptr;
 }
+
+int ah_pending(const oa_handle* h) { return h->ah ? h->ah->pending : 0; }
 
 }  // namespace oax
 
@@ -1000,12 +1007,17 @@ struct Job {
   bool done;
 };
 
-// The searches of `jobs` in lockstep on stream st: every round integrates
-// the speculative shots of all unfinished searches in one launch.
-int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
+// First part of a search: the point records of every job's spacetime, formed
+// on its stream from the current state (the only part that reads the state),
+// and the start of each job's walk.
+int search_prepare(std::vector<Job>& jobs) {
   for (Job& j : jobs) {
     AH_RC(ah_check(j.h, j.k));
     if (!j.h->ah) j.h->ah = new AhState();
+    if (j.h->ah->pending) {
+      oa_set_error("oa_ahfind: a begun search of this spacetime has not ended");
+      return OA_ERR_STATE;
+    }
   }
   for (Job& j : jobs) {
     AhState& a = *j.h->ah;
@@ -1032,6 +1044,13 @@ int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
     }
     j.done = false;
   }
+  return OA_OK;
+}
+
+// Second part: the walks of `jobs` in lockstep on stream st (every round
+// integrates the speculative shots of all unfinished searches in one launch)
+// and their results.
+int search_run(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
   AhState& buf = *jobs[0].h->ah;
   std::vector<double> want;
   std::vector<ShotIn> in;
@@ -1097,6 +1116,25 @@ int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
   return OA_OK;
 }
 
+int search_many(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
+  AH_RC(search_prepare(jobs));
+  return search_run(jobs, st, out);
+}
+
+// A begun search: its jobs run their walks on the search stream of the first
+// job's spacetime, which waits for the point records.
+int search_begin(std::vector<Job>& jobs, hipStream_t state_stream, oa_ahsearch** out) {
+  AH_RC(search_prepare(jobs));
+  AhState& a = *jobs[0].h->ah;
+  if (!a.search_stream) AH_HIP(hipStreamCreateWithFlags(&a.search_stream, hipStreamNonBlocking));
+  if (!a.fields_done) AH_HIP(hipEventCreateWithFlags(&a.fields_done, hipEventDisableTiming));
+  AH_HIP(hipEventRecord(a.fields_done, state_stream));
+  AH_HIP(hipStreamWaitEvent(a.search_stream, a.fields_done, 0));
+  for (Job& j : jobs) j.h->ah->pending += 1;
+  *out = reinterpret_cast<oa_ahsearch*>(new std::vector<Job>(std::move(jobs)));
+  return OA_OK;
+}
+
 }  // namespace
 
 extern "C" int oa_ahfind(oa_handle* h, int k, oa_horizon* out) {
@@ -1132,6 +1170,55 @@ extern "C" int oa_batch_ahfind(oa_batch* b, int n, oa_handle* const* members, co
     jobs[static_cast<size_t>(q)].k = k[q];
   }
   return search_many(jobs, members[0]->stream, out);
+}
+
+extern "C" int oa_ahfind_begin(oa_handle* h, int k, oa_ahsearch** search) {
+  if (!h || !search) {
+    oa_set_error("oa_ahfind_begin: null argument");
+    return OA_ERR_ARG;
+  }
+  *search = nullptr;
+  std::vector<Job> jobs(1);
+  jobs[0].h = h;
+  jobs[0].k = k;
+  return search_begin(jobs, h->stream, search);
+}
+
+extern "C" int oa_batch_ahfind_begin(oa_batch* b, int n, oa_handle* const* members, const int* k,
+                                     oa_ahsearch** search) {
+  if (!b || n <= 0 || !members || !k || !search) {
+    oa_set_error("oa_batch_ahfind_begin: bad argument");
+    return OA_ERR_ARG;
+  }
+  *search = nullptr;
+  std::vector<Job> jobs(static_cast<size_t>(n));
+  for (int q = 0; q < n; ++q) {
+    oa_handle* h = members[q];
+    if (!h || h->batch != b) {
+      oa_set_error("oa_batch_ahfind_begin: every spacetime must be a member of the batch");
+      return OA_ERR_ARG;
+    }
+    for (int r = 0; r < q; ++r)
+      if (members[r] == h) {
+        oa_set_error("oa_batch_ahfind_begin: a spacetime twice");
+        return OA_ERR_ARG;
+      }
+    jobs[static_cast<size_t>(q)].h = h;
+    jobs[static_cast<size_t>(q)].k = k[q];
+  }
+  return search_begin(jobs, members[0]->stream, search);
+}
+
+extern "C" int oa_ahfind_end(oa_ahsearch* search, oa_horizon* out) {
+  if (!search || !out) {
+    oa_set_error("oa_ahfind_end: null argument");
+    return OA_ERR_ARG;
+  }
+  std::vector<Job>* jobs = reinterpret_cast<std::vector<Job>*>(search);
+  for (Job& j : *jobs) j.h->ah->pending -= 1;
+  const int rc = search_run(*jobs, (*jobs)[0].h->ah->search_stream, out);
+  delete jobs;
+  return rc;
 }
 
 extern "C" int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n) {

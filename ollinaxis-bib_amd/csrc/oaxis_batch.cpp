@@ -134,6 +134,10 @@ extern "C" int oa_batch_remove(oa_batch* b, oa_handle* h) {
     oa_set_error("oa_batch_remove: not a member of this batch");
     return OA_ERR_ARG;
   }
+  if (ah_pending(h)) {
+    oa_set_error("oa_batch_remove: a begun horizon search of this spacetime has not ended (oa_ahfind_end)");
+    return OA_ERR_STATE;
+  }
   HIPCHK(hipStreamSynchronize(b->stream));
   h->stream = h->own_stream;
   h->batch = nullptr;

@@ -480,6 +480,10 @@ extern "C" int oa_create(const oa_params* pin, oa_handle** out) {
 
 extern "C" int oa_destroy(oa_handle* h) {
   if (!h) return OA_OK;
+  if (ah_pending(h)) {
+    oa_set_error("oa_destroy: a begun horizon search of this spacetime has not ended (oa_ahfind_end)");
+    return OA_ERR_STATE;
+  }
   if (h->batch) (void)oa_batch_remove(h->batch, h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (Level& L : h->L) {

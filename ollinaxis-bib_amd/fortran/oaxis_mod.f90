@@ -29,6 +29,7 @@ module oaxis_mod
             oa_batch_point_updates, oa_batch_launch_stats, oa_batch_set_kernel_timing, &
             oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule, &
             oa_constraints, oa_ahfind, oa_horizon_shape, oa_batch_ahfind, &
+            oa_ahfind_begin, oa_batch_ahfind_begin, oa_ahfind_end, &
             oa_checkpoint_size, oa_checkpoint_save, oa_checkpoint_load
 
   ! result of one apparent-horizon search (oa_horizon in include/oaxis.h)
@@ -283,6 +284,27 @@ module oaxis_mod
       integer(c_int), value :: n
       type(c_ptr), dimension(*) :: members
       integer(c_int), dimension(*) :: k
+      type(oa_horizon), dimension(*) :: res
+    end function
+    ! a horizon search in two calls: the begin reads the current state, the
+    ! end runs the shots beside whatever was enqueued in between
+    integer(c_int) function oa_ahfind_begin(h, k, search) bind(C, name="oa_ahfind_begin")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: k
+      type(c_ptr) :: search
+    end function
+    integer(c_int) function oa_batch_ahfind_begin(b, n, members, k, search) bind(C, name="oa_batch_ahfind_begin")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: n
+      type(c_ptr), dimension(*) :: members
+      integer(c_int), dimension(*) :: k
+      type(c_ptr) :: search
+    end function
+    integer(c_int) function oa_ahfind_end(search, res) bind(C, name="oa_ahfind_end")
+      import :: c_int, c_ptr, oa_horizon
+      type(c_ptr), value :: search
       type(oa_horizon), dimension(*) :: res
     end function
     ! 0D norms (max, min, nm1, nm2) of an evolved array, as save0Ddata forms them

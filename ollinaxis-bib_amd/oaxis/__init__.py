@@ -246,6 +246,15 @@ class Spacetime:
             out["theta"], out["rad"] = th[:n.value], rd[:n.value]
         return out
 
+    def ahfind_begin(self, k=0):
+        """Begin the search of horizon k on the current state; the returned
+        search's end() gives what ahfind would have given now, after the
+        caller has enqueued more work (a coarse step) to run beside its
+        shots."""
+        h = ctypes.c_void_p()
+        _check(lib().oa_ahfind_begin(self._h, int(k), ctypes.byref(h)), "oa_ahfind_begin")
+        return HorizonSearch(h, 1, single=True)
+
     def norms0d(self, name, level=0):
         """save0Ddata's norms of an evolved array: dict(max, min, nm1, nm2)."""
         out = (ctypes.c_double * 4)()
@@ -276,6 +285,27 @@ class Spacetime:
         v = ctypes.c_double()
         _check(lib().oa_point_updates(self._h, ctypes.byref(v)), "oa_point_updates")
         return v.value
+
+
+class HorizonSearch:
+    """A begun horizon search (oa_ahfind_begin / oa_batch_ahfind_begin):
+    end() runs its shots and returns the results (a dict for a spacetime's
+    search, a list for a batch's); it must be called exactly once, before
+    the spacetimes are closed or leave their batch."""
+
+    def __init__(self, h, n, single=False):
+        self._h, self._n, self._single = h, n, single
+
+    def end(self):
+        if self._n == 0:
+            return []
+        if self._h is None:
+            raise OaxisError("this horizon search has already ended")
+        res = (_abi.Horizon * self._n)()
+        h, self._h = self._h, None
+        _check(lib().oa_ahfind_end(h, res), "oa_ahfind_end")
+        out = [{name: getattr(r, name) for name, _ in _abi.Horizon._fields_} for r in res]
+        return out[0] if self._single else out
 
 
 class Batch:
@@ -324,6 +354,18 @@ class Batch:
         res = (_abi.Horizon * n)()
         _check(lib().oa_batch_ahfind(self._b, n, hs, ks, res), "oa_batch_ahfind")
         return [{name: getattr(r, name) for name, _ in _abi.Horizon._fields_} for r in res]
+
+    def ahfind_begin(self, searches):
+        """ahfind in two parts (HorizonSearch.end() returns the list)."""
+        n = len(searches)
+        if n == 0:
+            return HorizonSearch(None, 0)
+        hs = (ctypes.c_void_p * n)(*[st._h.value if isinstance(st._h, ctypes.c_void_p) else st._h
+                                     for st, _ in searches])
+        ks = (ctypes.c_int * n)(*[int(k) for _, k in searches])
+        h = ctypes.c_void_p()
+        _check(lib().oa_batch_ahfind_begin(self._b, n, hs, ks, ctypes.byref(h)), "oa_batch_ahfind_begin")
+        return HorizonSearch(h, n)
 
     def last_elapsed_ms(self):
         v = ctypes.c_double()

@@ -122,6 +122,9 @@ def load_library(path=LIBPATH):
         "oa_ahfind": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(Horizon)]),
         "oa_horizon_shape": (ctypes.c_int, [H, ctypes.c_int, dp, dp, ctypes.c_int, ip]),
         "oa_batch_ahfind": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(H), ip, ctypes.POINTER(Horizon)]),
+        "oa_ahfind_begin": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(H)]),
+        "oa_batch_ahfind_begin": (ctypes.c_int, [H, ctypes.c_int, ctypes.POINTER(H), ip, ctypes.POINTER(H)]),
+        "oa_ahfind_end": (ctypes.c_int, [H, ctypes.POINTER(Horizon)]),
         "oa_checkpoint_size": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_size_t)]),
         "oa_checkpoint_save": (ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_size_t]),
         "oa_checkpoint_load": (ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_size_t]),

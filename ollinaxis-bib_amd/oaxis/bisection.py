@@ -31,14 +31,17 @@ def critical_bisection(text_of, lo, hi, members=8, rounds=3, steps=200, ahfind_e
         amps = np.linspace(lo, hi, members)
         reqs = [Request(q, text_of(float(a)) + "\nahfind = .true.\nahfind_every = %d\n" % ahfind_every, steps,
                         0.0, "A=%.10g" % a) for q, a in enumerate(amps)]
-        step_cb = None
+        step_cb = admit_cb = None
         if progress is not None:
             def step_cb(k, n, rnd=rnd):
                 progress("round %d: coarse step %d, %d evolutions running" % (rnd, k, n))
+
+            def admit_cb(req, n, rnd=rnd):
+                progress("round %d: %s prepared (%d of %d)" % (rnd, req.kind, n, members))
         # every amplitude of a round starts together and the round advances
         # them in lockstep
         server = Server(memory_budget, check_every=ahfind_every, prepare_threads=prepare_threads, gang=True)
-        records, summary = server.run(reqs, step_progress=step_cb, step_every=ahfind_every)
+        records, summary = server.run(reqs, step_progress=step_cb, step_every=ahfind_every, admit_progress=admit_cb)
         outcome = [r.reason == "horizon" for r in records]
         total_updates += summary["point_updates"]
         if progress is not None:
@@ -47,7 +50,8 @@ def critical_bisection(text_of, lo, hi, members=8, rounds=3, steps=200, ahfind_e
         history.append(dict(amplitudes=amps.tolist(), horizon=outcome,
                             steps=[r.steps for r in records], reasons=[r.reason for r in records],
                             masses=[r.horizon.get("mass", 0.0) if r.horizon else 0.0 for r in records],
-                            wall_s=summary["wall_s"], p50_s=summary["p50_s"], p99_s=summary["p99_s"]))
+                            wall_s=summary["wall_s"], p50_s=summary["p50_s"], p99_s=summary["p99_s"],
+                            first_step_s=summary["first_step_s"]))
         # the first amplitude that forms a horizon, with every smaller one not
         # forming one, brackets the threshold
         try:

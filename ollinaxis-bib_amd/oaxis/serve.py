@@ -175,11 +175,13 @@ class Server:
             return "failed"          # a NaN in the state: the request retires with that status
         return None
 
-    def run(self, requests, clock=time.perf_counter, progress=None, step_progress=None, step_every=16):
+    def run(self, requests, clock=time.perf_counter, progress=None, step_progress=None, step_every=16,
+            admit_progress=None):
         """Serve ``requests`` (arrival times relative to the start); returns
         (records, summary).  progress(record) is called at each retirement,
         step_progress(coarse steps, active members) every step_every coarse
-        steps of the batch."""
+        steps of the batch, admit_progress(request, active members) when a
+        prepared request joins the batch."""
         oaxis = self.oaxis
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active, done = [], []
@@ -194,6 +196,8 @@ class Server:
         t0 = clock()
         total_updates = 0.0
         batch_steps = 0
+        search = None             # (members, HorizonSearch) begun after the last step
+        first_step = None         # time of the first coarse step
         try:
             while pending or active or preparing:
                 now = clock() - t0
@@ -214,6 +218,8 @@ class Server:
                 while preparing and (preparing[0][3].done() or not active or self.gang):
                     req, t_adm, fp, fut = preparing.pop(0)
                     self._admit(req, t_adm, batch, active, used, prepared=fut.result())
+                    if admit_progress is not None:
+                        admit_progress(req, len(active))
                 if not active:
                     if pending and not preparing:
                         time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
@@ -226,13 +232,24 @@ class Server:
                     done.append(rec)
                 if not active:
                     continue
+                if first_step is None:
+                    first_step = clock() - t0
                 batch.coarse_step()
                 batch_steps += 1
                 for a in active:
                     a["steps"] += 1
                 if step_progress is not None and batch_steps % step_every == 0:
                     step_progress(batch_steps, len(active))
-                retire = self._due_retirements(batch, active)
+                # the searches begun after the previous step ran beside this one
+                found = self._end_searches(search)
+                search = None
+                retire = self._due_retirements(batch, active, found)
+                due = [a for a in active if self._search_due(a) and all(a is not r for r, _ in retire)]
+                if due:
+                    try:
+                        search = (due, batch.ahfind_begin([(a["st"], 0) for a in due]))
+                    except oaxis.OaxisError:
+                        retire.extend((a, "failed") for a in due)
                 if retire:
                     batch.sync()
                     now = clock() - t0
@@ -245,6 +262,11 @@ class Server:
                         if progress is not None:
                             progress(rec)
         finally:
+            if search is not None:
+                try:
+                    search[1].end()
+                except Exception:
+                    pass
             for _, _, _, fut in preparing:
                 try:
                     fut.result()[1].close()
@@ -254,16 +276,43 @@ class Server:
             for a in active:
                 a["st"].close()
             batch.close()
-        return self._summary(done, clock() - t0, total_updates)
+        records, summary = self._summary(done, clock() - t0, total_updates)
+        summary["first_step_s"] = first_step
+        return records, summary
 
-    def _due_retirements(self, batch, active):
-        """Members to retire after a coarse step: final step, a check every
-        check_every steps (lapse collapse, dispersal, NaN), and the horizon
-        searches due at this step, run as one lockstep batch."""
-        oaxis = self.oaxis
+    def _search_due(self, a):
+        """A horizon search of member a is due after its current step."""
+        return (a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0
+                and a["steps"] * a["dt0"] >= a["ah_after"])
+
+    def _end_searches(self, search):
+        """Results of the searches begun after the previous step (they ran
+        beside this step): {id(member): "horizon" | "failed"} for the members
+        whose search found a horizon or failed."""
+        if search is None:
+            return {}
+        members, hs = search
+        try:
+            res = hs.end()
+        except self.oaxis.OaxisError:
+            return {id(a): "failed" for a in members}
+        out = {}
+        for a, r in zip(members, res):
+            if r["found"]:
+                a["rec"].horizon = r
+                out[id(a)] = "horizon"
+        return out
+
+    def _due_retirements(self, batch, active, found):
+        """Members to retire after a coarse step: a horizon found by the
+        search of the state one step back (the search runs beside the next
+        step, and the member retires after it), the final step, and a check
+        every check_every steps (lapse collapse, dispersal, NaN)."""
         retire = []
-        due = []
         for a in active:
+            if id(a) in found:
+                retire.append((a, found[id(a)]))
+                continue
             if a["steps"] >= a["req"].steps:
                 retire.append((a, "final_time"))
                 continue
@@ -271,21 +320,6 @@ class Server:
                 reason = self._check(a)
                 if reason:
                     retire.append((a, reason))
-                    continue
-            if (a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0
-                    and a["steps"] * a["dt0"] >= a["ah_after"]):
-                due.append(a)
-        if due:
-            try:
-                found = batch.ahfind([(a["st"], 0) for a in due])
-            except oaxis.OaxisError:
-                found = None
-            for q, a in enumerate(due):
-                if found is None:
-                    retire.append((a, "failed"))
-                elif found[q]["found"]:
-                    a["rec"].horizon = found[q]
-                    retire.append((a, "horizon"))
         return retire
 
     @staticmethod
@@ -320,10 +354,17 @@ class Server:
                     if st.ahfind(0)["found"]:
                         a["watch"] = False
                 reason = None
+                search = None
                 while reason is None:
                     st.coarse_step()
                     a["steps"] += 1
-                    reason = self._plain_retirement(a)
+                    reason = self._plain_retirement(a, search)
+                    search = None
+                    if reason is None and self._search_due(a):
+                        try:
+                            search = ([a], st.ahfind_begin(0))
+                        except self.oaxis.OaxisError:
+                            reason = "failed"
                 rec = a["rec"]
                 rec.retired = clock() - t0
                 rec.arrival = 0.0
@@ -339,22 +380,24 @@ class Server:
                 batch.close()
         return self._summary(done, clock() - t0, total_updates)
 
-    def _plain_retirement(self, a):
-        st = a["st"]
+    def _plain_retirement(self, a, search):
+        """As _due_retirements for a spacetime served alone; search is the
+        one begun after its previous step."""
+        if search is not None:
+            members, hs = search
+            try:
+                r = hs.end()
+            except self.oaxis.OaxisError:
+                return "failed"
+            if r["found"]:
+                a["rec"].horizon = r
+                return "horizon"
         if a["steps"] >= a["req"].steps:
             return "final_time"
         if self.check_every > 0 and a["steps"] % self.check_every == 0:
             reason = self._check(a)
             if reason:
                 return reason
-        if a["ah"] and a["watch"] and a["steps"] % a["ah_every"] == 0 and a["steps"] * a["dt0"] >= a["ah_after"]:
-            try:
-                h = st.ahfind(0)
-            except self.oaxis.OaxisError:
-                return "failed"
-            if h["found"]:
-                a["rec"].horizon = h
-                return "horizon"
         return None
 
 

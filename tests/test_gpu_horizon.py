@@ -203,9 +203,10 @@ Nztotal = 64
 
 def test_serving_retires_on_horizon(oaxis_mod):
     """With ahafter > 0 the search at t = 0 is skipped, so the puncture's
-    horizon counts as formed at the first search after ahafter: the request
-    retires there with reason "horizon"; with ahafter = 0 the horizon exists
-    from t = 0 and the request runs to its last step."""
+    horizon counts as formed at the first search after ahafter: the search
+    runs beside the next coarse step and the request retires after it with
+    reason "horizon"; with ahafter = 0 the horizon exists from t = 0 and the
+    request runs to its last step."""
     oaxis = oaxis_mod
     from oaxis.serve import Request, Server
     late = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\nahafter = 0.1\n")
@@ -215,7 +216,7 @@ def test_serving_retires_on_horizon(oaxis_mod):
     records, summary = Server(1e12, check_every=0).run(reqs)
     by = {r.rid: r for r in records}
     first = next(s for s in range(2, 13, 2) if s * dt0 >= 0.1)
-    assert by[0].reason == "horizon" and by[0].steps == first, (by[0].reason, by[0].steps, first)
+    assert by[0].reason == "horizon" and by[0].steps == first + 1, (by[0].reason, by[0].steps, first)
     assert by[0].horizon["found"] == 1 and abs(by[0].horizon["mass"] - 2.0) < 1e-3
     assert by[1].reason == "final_time" and by[1].steps == 6
 
@@ -252,8 +253,9 @@ boundtype = radiative
 def test_scalar_collapse_forms_a_horizon(oaxis_mod, oracle_mod):
     """A strong scalar pulse (Gamma-driver, 2 levels) collapses: searching
     every 2 coarse steps, device and oracle agree on every search and find
-    the horizon at the same step; the serving loop retires the request
-    there with reason "horizon"."""
+    the horizon at the same step; the serving loop, whose search runs beside
+    the next coarse step, retires the request after that step with reason
+    "horizon" and the search's result."""
     oaxis = oaxis_mod
     p = oaxis.params_from_text(COLLAPSE)
     orc = oracle_mod.Oracle(p)
@@ -282,7 +284,7 @@ def test_scalar_collapse_forms_a_horizon(oaxis_mod, oracle_mod):
     from oaxis.serve import Request, Server
     records, _ = Server(1e12, check_every=0).run(
         [Request(0, COLLAPSE + "ahfind = .true.\nahfind_every = 2\n", 60, 0.0, "collapse")])
-    assert records[0].reason == "horizon" and records[0].steps == formed[0]
+    assert records[0].reason == "horizon" and records[0].steps == formed[0] + 1
     assert abs(records[0].horizon["mass"] - formed[1]) < 1e-9
 
 
@@ -325,3 +327,30 @@ def test_moving_centres_match_oracle(oaxis_mod, oracle_mod, k):
     orc.close()
     assert dev["found"] == 1 and abs(abs(dev["zcenter"]) - 3.0) < h
     compare(dev, ref, "ahmove k=%d" % k)
+
+
+def test_begun_search_matches_a_direct_search(oaxis_mod):
+    """oa_ahfind_begin / oa_ahfind_end with a coarse step enqueued between
+    them give what oa_ahfind gives on the state at the begin: the search
+    reads only what the begin formed, and its shots run beside the step.
+    A spacetime with a begun search cannot begin another before it ends."""
+    oaxis = oaxis_mod
+    text = _schwarzschild_text(2.0, "")
+    p = oaxis.params_from_text(text)
+    with oaxis.Spacetime(p) as a, oaxis.Spacetime(p) as b:
+        for st in (a, b):
+            st.initial_data()
+            st.initial_aux()
+            st.evolve(2)
+        direct = a.ahfind(0)
+        search = b.ahfind_begin(0)
+        with pytest.raises(oaxis.OaxisError):
+            b.ahfind_begin(0)
+        b.coarse_step()
+        begun = search.end()
+        for key in ("found", "shots", "ntheta", "zinit", "area", "mass"):
+            assert begun[key] == direct[key], (key, begun[key], direct[key])
+        # the step ran: the next search sees the evolved state
+        a.coarse_step()
+        assert b.ahfind(0)["zinit"] == a.ahfind(0)["zinit"]
+
