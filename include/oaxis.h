@@ -124,6 +124,9 @@ typedef struct oa_params {
   double ahafter;
   int    ahmove, N_ah, ahmethod;  /* ahmethod: OA_AH_*                     */
   double z_ah1, z_ah2, ahzinit, ahrmax, ahepsilon_shoot;
+  /* spectral method, fast flow (param.f90:549-567) */
+  int    ahNmodes, ahmaxiter, ahshootfirst;
+  double ahepsilon_spectral, ahAA, ahBB, aheccentric;
   /* Brill waves (param.f90:572-576) */
   double brill_a0, brill_sr0, brill_sz0;
   /* matter (param.f90:615-648) */
@@ -323,25 +326,43 @@ typedef struct oa_horizon {
   double area;       /* ah_area: 0 unless found; doubled with eqsym (:989-995)  */
   double radius;     /* RAH = sqrt(area/pi)/2 (:997)                            */
   double mass;       /* MAH = RAH/2: no angular momentum in scope (:1065-1076)  */
+  int    method;     /* OA_AH_* of the search.  With spectral and both the
+                        fields above describe the fast flow (ahspectral,
+                        :1113-1972): ntheta = Ntheta = 10 ahNmodes, errorflag
+                        its error flag (also set when ac(0) < dr0 + dz0), and
+                        shots, zinit, dradlast those of the shooting search
+                        that ran before it in this call (both; ahshootfirst
+                        on the first call), else 0.                          */
+  int    iterations; /* fast-flow iterations (0 for shoot)                      */
+  double residual;   /* max_j |c_j| of the last iteration (0 for shoot)         */
 } oa_horizon;
 
 /* horizon_finder (horizon_finder.f90:2-237) for horizon k (0 <= k < N_ah,
- * centre 0, z_ah1, z_ah2) with ahmethod = shoot, on the current state of
- * box 0 (every level projected): replaces the reference's call at
- * evolve.f90:251 and :429-432.  The metric quantities the horizon ODE
- * interpolates are formed on the device for every level, and the shots of
- * the search (the march along the axis and the bisection) are integrated on
- * the device, many per launch; the sequence of shots and every decision
- * between them are the reference's.  With ahmove the centres of horizons 1
- * and 2 first move to the nearest maximum of phi on the axis (:117-180).
- * The spectral method returns OA_ERR_UNSUPPORTED. */
+ * centre 0, z_ah1, z_ah2) on the current state of box 0 (every level
+ * projected): replaces the reference's call at evolve.f90:251 and
+ * :429-432.  The metric quantities the horizon ODE interpolates are formed
+ * on the device for every level.  ahmethod = shoot: the shots of the search
+ * (the march along the axis and the bisection) are integrated on the
+ * device, many per launch; the sequence of shots and every decision between
+ * them are the reference's.  ahmethod = spectral: the fast flow of the
+ * cosine coefficients (ahspectral, :1113-1972) runs whole on the device, one
+ * workgroup per search, from the last horizon found (ack(k,:), kept between
+ * searches) or a large circle (an ellipse with aheccentric), after a
+ * shooting search on the first call with ahshootfirst (:1292-1303);
+ * ahmethod = both: a shooting search, then the flow from its surface
+ * (:223-226).  With ahmove the centres of horizons 1 and 2 first move to
+ * the nearest maximum of phi on the axis (:117-180). */
 int oa_ahfind(oa_handle* h, int k, oa_horizon* out);
-/* (theta_i, h(theta_i)), i = 0..ntheta, of the last integration of horizon k
- * (what ahshoot writes to ah_radius_SH, :956-966; h = 0 when not found).
- * cap is the room in theta and rad; *n receives ntheta + 1 (also when the
- * arrays are null, to size them).  The integration is run again on the
- * quantities the search interpolated, so it needs no state; a later search
- * of the same spacetime replaces them (OA_ERR_STATE: search again). */
+/* (theta_i, h(theta_i)), i = 0..ntheta, of horizon k's last search, h = 0
+ * when not found.  Shooting: the last integration (what ahshoot writes to
+ * ah_radius_SH, :956-966), run again on the quantities the search
+ * interpolated, so it needs no state; a later search of the same spacetime
+ * replaces them (OA_ERR_STATE: search again).  Spectral and both: the
+ * surface of the flow's last iteration on theta_i = i pi / (s Ntheta),
+ * s = 2 with equatorial symmetry about a centre at 0, else 1 (what
+ * ahspectral writes to ah_radius_SP, :1766-1790).  cap is the room in
+ * theta and rad; *n receives ntheta + 1 (also when the arrays are null, to
+ * size them). */
 int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad, int cap, int* n);
 /* oa_ahfind in two calls, so that a search can run while the spacetime
  * evolves on: oa_ahfind_begin forms the quantities the search interpolates

@@ -93,6 +93,6 @@ int plan_level(Plan& plan, const std::vector<oa_handle*>& all, int level);
 void ah_release(oa_handle* h);                       // free the horizon finder's device memory
 int ah_pending(const oa_handle* h);                  // begun horizon searches not yet ended
 int ah_record_doubles();                             // horizon finder: doubles per grid point
-double ah_fixed_bytes(int n_ah);                     // horizon finder: per-handle device bytes besides those
+double ah_fixed_bytes(int n_ah, int method);                     // horizon finder: per-handle device bytes besides those
 
 }  // namespace oax

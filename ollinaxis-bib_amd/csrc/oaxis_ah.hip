@@ -615,6 +615,192 @@ __global__ void __launch_bounds__(kShotBlock) k_ah_dzphi(const AhCtx* __restrict
   if (threadIdx.x == 0) out[0] = v[AF_Dzphi];
 }
 
+// ------------------------------------------------------------ spectral finder
+// ahspectral (:1113-1972): the fast flow of the cosine coefficients of the
+// surface r = h(theta) (Gundlach, gr-qc/9707050).  One workgroup runs one
+// search to the end: each stage evaluates ah_source at the Ntheta + 1 points
+// of the surface (one point per half-wave, kSpecBlock / 32 points per
+// round), one lane per mode sums the trapezoid coefficients in the
+// reference's order, every lane rebuilds a slice of h and h' from the
+// coefficients, and lane 0 runs the residual and the step control.  The
+// surface, the coefficients and their previous values stay in LDS for the
+// whole search, and nothing returns to the host between iterations.
+constexpr int kSpecModes = 50;                      // ac(0:50) (:1165-1168)
+constexpr int kSpecThetaMax = 10 * kSpecModes;      // Ntheta = 10 Nmode (:1232)
+constexpr int kSpecBlock = 512;                     // 16 surface points per round
+
+struct SpecIn {
+  const AhCtx* ctx;
+  const double* cs;                   // cos(j theta_i), row j of ntheta + 1 values
+  const double* sn;                   // sin(j theta_i)
+  double* rad;                        // out: h(theta_i) of the last iteration
+  double zcenter, dtheta, AA, BB, eps;
+  int nmode, ntheta, step, maxiter;
+  int reconstruct;                    // 1: h, h' from ac; 0: the circle h = ac(0)/2
+  double ac[kSpecModes + 1];          // initial coefficients
+};
+
+struct SpecOut {
+  double ac[kSpecModes + 1];
+  double res, area;                   // residual; trapezoid of the last area elements
+  int iter, err;
+};
+
+__global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __restrict__ in,
+                                                            SpecOut* __restrict__ out) {
+  __shared__ double sv[kSpecBlock / 32 * AF_N];
+  __shared__ double rad[kSpecThetaMax + 1], drad[kSpecThetaMax + 1];
+  __shared__ double rad_p[kSpecThetaMax + 1], drad_p[kSpecThetaMax + 1];
+  __shared__ double Ifunc[kSpecThetaMax + 1], area[kSpecThetaMax + 1];
+  __shared__ double ac[kSpecModes + 1], cc[kSpecModes + 1], ac_old[kSpecModes + 1], cc_old[kSpecModes + 1];
+  __shared__ double x_res, x_AA;
+  __shared__ int x_err, x_rej;
+  const SpecIn& s = in[blockIdx.x];
+  const AhCtx& c = *s.ctx;
+  const int t = static_cast<int>(threadIdx.x);
+  const int nt = static_cast<int>(blockDim.x);
+  const int N = s.ntheta, nm = s.nmode, step = s.step;
+  const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
+  const double smallpi = M_PI;
+  const double dtheta = s.dtheta, BB = s.BB, AA0 = s.AA, zcenter = s.zcenter;
+  const double* __restrict__ cs = s.cs;
+  const double* __restrict__ sn = s.sn;
+  // h(theta_i), h'(theta_i) from the coefficients (:1320-1327, :1602-1609)
+  auto rebuild = [&]() {
+    for (int i = t; i <= N; i += nt) {
+      double r = half * ac[0], d = zero;
+      for (int j = step; j <= nm; j += step) {
+        r = r + ac[j] * cs[j * (N + 1) + i];
+        d = d - static_cast<double>(j) * ac[j] * sn[j * (N + 1) + i];
+      }
+      rad[i] = r;
+      drad[i] = d;
+    }
+  };
+  for (int j = t; j <= kSpecModes; j += nt) {
+    ac[j] = s.ac[j];
+    cc[j] = ac_old[j] = cc_old[j] = zero;
+  }
+  __syncthreads();
+  if (s.reconstruct) {
+    rebuild();
+  } else {
+    for (int i = t; i <= N; i += nt) {
+      rad[i] = half * ac[0];
+      drad[i] = zero;
+    }
+  }
+  for (int i = t; i <= N; i += nt) Ifunc[i] = area[i] = zero;
+  AhCell cell;
+  cell.q = nullptr;
+  cell.i0 = cell.j0 = 0;
+  const int hw = t >> 5, nhw = nt >> 5;
+  const int jm = t * step;                 // this lane's mode in the coefficient sums
+  double res = 1.0, AA = AA0;
+  int iter = 0, err = 0;
+  __syncthreads();
+  while (res > s.eps && iter < s.maxiter) {                                      // :1451
+    ++iter;
+    for (int j = t; j <= kSpecModes; j += nt) {
+      ac_old[j] = ac[j];
+      cc_old[j] = cc[j];
+    }
+    for (int i = t; i <= N; i += nt) {
+      rad_p[i] = rad[i];
+      drad_p[i] = drad[i];
+    }
+    if (t == 0) x_err = 0;
+    __syncthreads();
+    for (int q = 1; q <= 2; ++q) {
+      // sources at every point (:1494-1501); the rounds' spare half-waves
+      // evaluate the last point again and keep nothing
+      for (int base = 0; base <= N; base += nhw) {
+        const int i = base + hw;
+        const int ii = i <= N ? i : N;
+        double sdh, sarea;
+        const int e = ah_source(c, rad[ii], drad[ii], static_cast<double>(ii) * dtheta, zcenter, sdh, sarea, sv, cell);
+        if ((t & 31) == 0 && i <= N) {
+          Ifunc[i] = sdh;
+          area[i] = sarea;
+          if (e) x_err = 1;
+        }
+      }
+      __syncthreads();
+      err = x_err;
+      if (err) break;                                                            // goto 100
+      if (t == 0) {                                                              // :1506-1515
+        Ifunc[0] = Ifunc[1];
+        area[0] = zero;
+        if (step == 1) {
+          Ifunc[N] = Ifunc[N - 1];
+          area[N] = zero;
+        }
+      }
+      __syncthreads();
+      if (jm <= nm) {                                                            // :1539-1563, :1590-1592
+        const double* cj = cs + jm * (N + 1);
+        double cc_j = half * dtheta * Ifunc[0];
+        for (int i = 1; i < N; ++i) cc_j = cc_j + dtheta * Ifunc[i] * cj[i];
+        cc_j = cc_j + half * dtheta * Ifunc[N] * cj[N];
+        cc_j = two * static_cast<double>(step) * cc_j / smallpi;
+        const double jj = static_cast<double>(jm) * static_cast<double>(jm);
+        cc_j = cc_j + jj * ac[jm];
+        cc[jm] = cc_j;
+        if (q == 1) cc_old[jm] = cc_j;
+        ac[jm] = ac_old[jm] - half * AA * static_cast<double>(q) / (one + BB * jj) * cc_j;
+      }
+      __syncthreads();
+      rebuild();
+      __syncthreads();
+    }
+    if (err) break;
+    if (t == 0) {                                                                // :1636-1682
+      double r = zero, aux1 = zero, aux2 = zero;
+      for (int j = 0; j <= nm; j += step) r = std::fmax(r, std::fabs(cc[j]));
+      for (int j = 0; j <= nm; j += step) {
+        const double jj = static_cast<double>(j) * static_cast<double>(j);
+        const double tc = ac_old[j] - AA / (one + BB * jj) * cc_old[j];
+        aux1 = aux1 + std::fabs(ac[j] - tc);
+        aux2 = aux2 + std::fabs(ac_old[j]);
+      }
+      int rej = 0;
+      double AAn = AA;
+      if (aux1 > 0.05 * aux2) {
+        AAn = half * AA;
+        rej = 1;
+      } else if (aux1 < 0.001 * aux2) {
+        AAn = std::fmin(1.5 * AA, AA0);
+      }
+      x_res = r;
+      x_AA = AAn;
+      x_rej = rej;
+    }
+    __syncthreads();
+    res = x_res;
+    AA = x_AA;
+    if (x_rej) {
+      if (jm <= nm) ac[jm] = ac_old[jm];
+      for (int i = t; i <= N; i += nt) {
+        rad[i] = rad_p[i];
+        drad[i] = drad_p[i];
+      }
+    }
+    __syncthreads();
+  }
+  SpecOut& o = out[blockIdx.x];
+  for (int j = t; j <= kSpecModes; j += nt) o.ac[j] = ac[j];
+  for (int i = t; i <= N; i += nt) s.rad[i] = rad[i];
+  if (t == 0) {                                                                  // :1833-1839
+    double a = half * dtheta * area[0];
+    for (int i = 1; i < N; ++i) a = a + dtheta * area[i];
+    a = a + half * dtheta * area[N];
+    o.area = a;
+    o.res = res;
+    o.iter = iter;
+    o.err = err;
+  }
+}
+
 // ================================================================ host side
 int hip_rc(hipError_t e, const char* what) {
   if (e == hipSuccess) return OA_OK;
@@ -668,6 +854,23 @@ struct AhState {
   double last_z[3] = {0.0, 0.0, 0.0}, last_zc[3] = {0.0, 0.0, 0.0};
   int n_of[3] = {0, 0, 0};
   double* d_scalar = nullptr;             // one value back from the device
+  // spectral finder: ack(k,:) (shared with the shooting search, which
+  // projects its surface onto it), its own first-call flags, the surface of
+  // its last iteration per horizon, the cos / sin tables of the current
+  // (Nmode, step) and the staging of its launches
+  double ack[3][kSpecModes + 1] = {};
+  int first_sp[3] = {1, 1, 1};
+  int shape_spec[3] = {0, 0, 0};          // horizon k's last search ended with the fast flow
+  double spec_dtheta[3] = {0.0, 0.0, 0.0};
+  double* d_srad[3] = {nullptr, nullptr, nullptr};
+  double* d_cs = nullptr;
+  double* d_sn = nullptr;
+  int tab_nmode = 0, tab_step = 0;
+  SpecIn* h_sin = nullptr;                // pinned
+  SpecOut* h_sout = nullptr;              // pinned
+  SpecIn* d_sin = nullptr;
+  SpecOut* d_sout = nullptr;
+  int scap = 0;
   hipStream_t search_stream = nullptr;    // the shots of a begun search (oa_*ahfind_begin)
   hipEvent_t fields_done = nullptr;       // the point records of a begun search are formed
   int pending = 0;                        // a begun search awaits its end
@@ -897,10 +1100,15 @@ int build_fields(oa_handle* h, AhState& a) {
 int ah_record_doubles() { return AF_N; }
 
 // the context, the recorded last integration of each horizon and the shot
-// tables at their usual size (one march chunk)
-double ah_fixed_bytes(int n_ah) {
-  return static_cast<double>(sizeof(AhCtx)) + 2.0 * n_ah * sizeof(double) * (kAhNthetaMax + 1) +
-         static_cast<double>(kAhScanChunk) * 2.0 * (sizeof(ShotIn) + sizeof(ShotOut));
+// tables at their usual size (one march chunk); with the spectral finder
+// also its surfaces, trigonometric tables and one search's staging
+double ah_fixed_bytes(int n_ah, int method) {
+  double b = static_cast<double>(sizeof(AhCtx)) + 2.0 * n_ah * sizeof(double) * (kAhNthetaMax + 1) +
+             static_cast<double>(kAhScanChunk) * 2.0 * (sizeof(ShotIn) + sizeof(ShotOut));
+  if (method != OA_AH_SHOOT)
+    b += n_ah * sizeof(double) * (kSpecThetaMax + 1) + 2.0 * sizeof(double) * (kSpecModes + 1) * (kSpecThetaMax + 1) +
+         static_cast<double>(n_ah) * (sizeof(SpecIn) + sizeof(SpecOut));
+  return b;
 }
 
 void ah_release(oa_handle* h) {
@@ -918,6 +1126,14 @@ void ah_release(oa_handle* h) {
   if (a->d_out) (void)hipFree(a->d_out);
   if (a->h_in) (void)hipHostFree(a->h_in);
   if (a->h_out) (void)hipHostFree(a->h_out);
+  for (int k = 0; k < 3; ++k)
+    if (a->d_srad[k]) (void)hipFree(a->d_srad[k]);
+  if (a->d_cs) (void)hipFree(a->d_cs);
+  if (a->d_sn) (void)hipFree(a->d_sn);
+  if (a->d_sin) (void)hipFree(a->d_sin);
+  if (a->d_sout) (void)hipFree(a->d_sout);
+  if (a->h_sin) (void)hipHostFree(a->h_sin);
+  if (a->h_sout) (void)hipHostFree(a->h_sout);
   if (a->search_stream) (void)hipStreamDestroy(a->search_stream);
   if (a->fields_done) (void)hipEventDestroy(a->fields_done);
   delete a;
@@ -947,9 +1163,9 @@ int ah_check(const oa_handle* h, int k) {
     oa_set_error("oa_ahfind: horizon 1 must be above the equator and horizon 2 below");
     return OA_ERR_PARAM;
   }
-  if (p.ahmethod != OA_AH_SHOOT) {
-    oa_set_error("oa_ahfind: only ahmethod = shoot is supported");
-    return OA_ERR_UNSUPPORTED;
+  if (p.ahmethod != OA_AH_SHOOT && (p.ahNmodes < 1 || p.ahNmodes > kSpecModes)) {
+    oa_set_error("oa_ahfind: ahNmodes must lie in 1..50 for the spectral finder");
+    return OA_ERR_PARAM;
   }
   if (h->nlev > kAhMaxLevels) {
     oa_set_error("oa_ahfind: too many levels");
@@ -1005,6 +1221,12 @@ struct Job {
   Search s;
   double zstart;
   bool done;
+  // shooting searches this call runs: before the flow (ahmethod = both,
+  // :223-226) and inside it (ahshootfirst on the first call, :1292-1303)
+  bool shoot_first = false, shoot_in_flow = false, flow = false;
+  int shots = 0;                           // of the last shooting search
+  double zinit = 0.0, dradlast = 0.0;
+  SpecOut spec{};
 };
 
 // First part of a search: the point records of every job's spacetime, formed
@@ -1042,20 +1264,26 @@ int search_prepare(std::vector<Job>& jobs) {
     } else {
       j.zstart = (zcenter >= 0.0) ? 0.8 * L0.zmax : 0.8 * L0.zmin;
     }
-    j.done = false;
+    j.flow = p.ahmethod != OA_AH_SHOOT;
+    j.shoot_first = p.ahmethod != OA_AH_SPECTRAL;
+    j.shoot_in_flow = j.flow && a.first_sp[j.k] && p.ahshootfirst;
+    j.done = !j.shoot_first;
   }
   return OA_OK;
 }
 
-// Second part: the walks of `jobs` in lockstep on stream st (every round
-// integrates the speculative shots of all unfinished searches in one launch)
-// and their results.
-int search_run(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
+// The walks of the jobs not yet done, in lockstep on stream st (every round
+// integrates the speculative shots of all unfinished searches in one
+// launch), then each one's outcome: ahfound(k) and where to integrate its
+// last shot again.
+int run_walks(std::vector<Job>& jobs, hipStream_t st) {
   AhState& buf = *jobs[0].h->ah;
   std::vector<double> want;
   std::vector<ShotIn> in;
   std::vector<std::pair<size_t, double>> owner;   // (job, zinit) of every launched shot
   std::vector<ShotRes> res;
+  std::vector<bool> ran(jobs.size());
+  for (size_t q = 0; q < jobs.size(); ++q) ran[q] = !jobs[q].done;
   for (;;) {
     in.clear();
     owner.clear();
@@ -1080,38 +1308,262 @@ int search_run(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
     AH_RC(run_shots(st, buf, in, res));
     for (size_t t = 0; t < in.size(); ++t) jobs[owner[t].first].s.table[key_of(owner[t].second)] = res[t];
   }
-  // where to integrate the last shot again when its points are asked for
-  for (Job& j : jobs) {
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    if (!ran[q]) continue;
+    Job& j = jobs[q];
     AhState& a = *j.h->ah;
+    a.found[j.k] = j.s.found;
+    j.shots = j.s.shots;
+    j.zinit = j.s.zinit;
+    j.dradlast = j.s.dradlast;
+    // where to integrate the last shot again when its points are asked for
     a.gen_of[j.k] = j.s.shots > 0 ? a.gen : -1;
     a.last_z[j.k] = j.s.last_zinit;
     a.last_zc[j.k] = j.s.zcenter;
     a.n_of[j.k] = j.s.shots > 0 ? j.s.last.ntheta + 1 : 0;
+    a.shape_spec[j.k] = 0;
   }
+  return OA_OK;
+}
+
+// ahshoot's projection of its last surface onto ack(k,0..20) (:872-910),
+// for every job whose shooting search just found a horizon: the last
+// integration of each runs again (one launch) recording its points, and
+// the trapezoids over those uneven points run here in the reference's order.
+int project_shots(std::vector<Job*>& js, hipStream_t st) {
+  if (js.empty()) return OA_OK;
+  AhState& buf = *js[0]->h->ah;
+  std::vector<ShotIn> in;
+  for (Job* j : js) {
+    AhState& a = *j->h->ah;
+    if (!a.d_theta[j->k]) {
+      AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_theta[j->k]), sizeof(double) * (kAhNthetaMax + 1)));
+      AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_rad[j->k]), sizeof(double) * (kAhNthetaMax + 1)));
+    }
+    in.push_back(ShotIn{a.dctx, a.last_z[j->k], a.last_zc[j->k], a.d_theta[j->k], a.d_rad[j->k]});
+  }
+  std::vector<ShotRes> res;
+  AH_RC(run_shots(st, buf, in, res));
+  std::vector<double> th, rd;
+  const double half = 0.5, two = 2.0, smallpi = M_PI;
+  for (size_t q = 0; q < js.size(); ++q) {
+    Job& j = *js[q];
+    AhState& a = *j.h->ah;
+    const int n = res[q].ntheta;
+    if (n + 1 != a.n_of[j.k]) {
+      oa_set_error("oa_ahfind: the repeated integration differs from the search's");
+      return OA_ERR_STATE;
+    }
+    th.resize(static_cast<size_t>(n) + 1);
+    rd.resize(static_cast<size_t>(n) + 1);
+    AH_HIP(hipMemcpyAsync(th.data(), a.d_theta[j.k], sizeof(double) * (n + 1), hipMemcpyDeviceToHost, st));
+    AH_HIP(hipMemcpyAsync(rd.data(), a.d_rad[j.k], sizeof(double) * (n + 1), hipMemcpyDeviceToHost, st));
+    AH_HIP(hipStreamSynchronize(st));
+    const int iaux = (j.h->p.eqsym && j.s.zcenter == 0.0) ? 2 : 1;
+    double* ack = a.ack[j.k];
+    for (int m = 0; m <= kSpecModes; ++m) ack[m] = 0.0;
+    for (int m = 0; m <= 20; m += iaux) {
+      for (int i = 0; i < n; ++i)
+        ack[m] = ack[m] + half * (th[i + 1] - th[i]) *
+                              (rd[i + 1] * std::cos(static_cast<double>(m) * th[i + 1]) +
+                               rd[i] * std::cos(static_cast<double>(m) * th[i]));
+      ack[m] = two * static_cast<double>(iaux) * ack[m] / smallpi;
+    }
+  }
+  return OA_OK;
+}
+
+int spec_capacity(AhState& a, int n) {
+  if (n <= a.scap) return OA_OK;
+  if (a.h_sin) (void)hipHostFree(a.h_sin);
+  if (a.h_sout) (void)hipHostFree(a.h_sout);
+  if (a.d_sin) (void)hipFree(a.d_sin);
+  if (a.d_sout) (void)hipFree(a.d_sout);
+  a.h_sin = nullptr;
+  a.h_sout = nullptr;
+  a.d_sin = nullptr;
+  a.d_sout = nullptr;
+  a.scap = 0;
+  AH_HIP(hipHostMalloc(reinterpret_cast<void**>(&a.h_sin), sizeof(SpecIn) * n, hipHostMallocDefault));
+  AH_HIP(hipHostMalloc(reinterpret_cast<void**>(&a.h_sout), sizeof(SpecOut) * n, hipHostMallocDefault));
+  AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_sin), sizeof(SpecIn) * n));
+  AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_sout), sizeof(SpecOut) * n));
+  a.scap = n;
+  return OA_OK;
+}
+
+// cos(j theta_i) and sin(j theta_i), theta_i = i pi / (step Ntheta), as the
+// reference evaluates them (theta(i) first, then j times it)
+int spec_tables(AhState& a, int nmode, int step, hipStream_t st) {
+  if (a.d_cs && a.tab_nmode == nmode && a.tab_step == step) return OA_OK;
+  const int N = 10 * nmode;
+  const double dtheta = M_PI / static_cast<double>(step * N);
+  std::vector<double> c(static_cast<size_t>(nmode + 1) * (N + 1)), sn(c.size());
+  for (int j = 0; j <= nmode; ++j)
+    for (int i = 0; i <= N; ++i) {
+      const double th = static_cast<double>(i) * dtheta;
+      c[static_cast<size_t>(j) * (N + 1) + i] = std::cos(static_cast<double>(j) * th);
+      sn[static_cast<size_t>(j) * (N + 1) + i] = std::sin(static_cast<double>(j) * th);
+    }
+  if (!a.d_cs) {
+    const size_t bytes = sizeof(double) * (kSpecModes + 1) * (kSpecThetaMax + 1);
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_cs), bytes));
+    AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_sn), bytes));
+  }
+  AH_HIP(hipMemcpyAsync(a.d_cs, c.data(), sizeof(double) * c.size(), hipMemcpyHostToDevice, st));
+  AH_HIP(hipMemcpyAsync(a.d_sn, sn.data(), sizeof(double) * sn.size(), hipMemcpyHostToDevice, st));
+  AH_HIP(hipStreamSynchronize(st));
+  a.tab_nmode = nmode;
+  a.tab_step = step;
+  return OA_OK;
+}
+
+// The fast flow of every job that runs it, one workgroup each, in one
+// launch: the starting surface (:1284-1407) here, the iterations on the
+// device, then ahfound(k) and ack(k,:) (:1700-1733).
+int run_flows(std::vector<Job*>& js, hipStream_t st) {
+  if (js.empty()) return OA_OK;
+  AhState& buf = *js[0]->h->ah;
+  AH_RC(spec_capacity(buf, static_cast<int>(js.size())));
+  const double half = 0.5, one = 1.0, two = 2.0, smallpi = M_PI;
+  for (size_t q = 0; q < js.size(); ++q) {
+    Job& j = *js[q];
+    AhState& a = *j.h->ah;
+    const oa_params& p = j.h->p;
+    const double zcenter = j.s.zcenter;
+    const int nmode = p.ahNmodes, N = 10 * nmode;
+    const int step = (p.eqsym && zcenter == 0.0) ? 2 : 1;                  // :1250-1258
+    const double dtheta = smallpi / static_cast<double>(step * N);
+    AH_RC(spec_tables(a, nmode, step, st));
+    if (!a.d_srad[j.k]) AH_HIP(hipMalloc(reinterpret_cast<void**>(&a.d_srad[j.k]), sizeof(double) * (kSpecThetaMax + 1)));
+    SpecIn& in = buf.h_sin[q];
+    std::memset(&in, 0, sizeof in);
+    in.ctx = a.dctx;
+    in.cs = a.d_cs;
+    in.sn = a.d_sn;
+    in.rad = a.d_srad[j.k];
+    in.zcenter = zcenter;
+    in.dtheta = dtheta;
+    in.AA = p.ahAA;
+    in.BB = p.ahBB;
+    in.eps = p.ahepsilon_spectral;
+    in.nmode = nmode;
+    in.ntheta = N;
+    in.step = step;
+    in.maxiter = p.ahmaxiter;
+    if (a.found[j.k]) {                                                      // :1311-1330
+      for (int m = 0; m <= kSpecModes; ++m) in.ac[m] = a.ack[j.k][m];
+      in.reconstruct = 1;
+    } else {                                                                 // :1340-1356
+      const double ahrmax = (p.ahrmax > 0.9) ? 0.9 : p.ahrmax;
+      if (zcenter == 0.0) in.ac[0] = (p.ahzinit == 0.0) ? 1.8 * ahrmax * a.hctx.L[0].zmax : 2.0 * p.ahzinit;
+      else in.ac[0] = 1.95 * std::fabs(zcenter);
+      in.reconstruct = 0;
+      if (zcenter != 0.0 && p.aheccentric != 0.0) {                         // :1365-1407
+        std::vector<double> rad(static_cast<size_t>(N) + 1);
+        for (int i = 0; i <= N; ++i) {
+          const double e = p.aheccentric * std::sin(static_cast<double>(i) * dtheta);
+          rad[i] = half * in.ac[0] / std::sqrt(one - e * e);
+        }
+        for (int m = 0; m <= nmode; ++m) {
+          double c = half * dtheta * rad[0];
+          for (int i = 1; i < N; ++i) c = c + dtheta * rad[i] * std::cos(static_cast<double>(m) * (static_cast<double>(i) * dtheta));
+          c = c + half * dtheta * rad[N] * std::cos(static_cast<double>(m) * (static_cast<double>(N) * dtheta));
+          in.ac[m] = two * static_cast<double>(step) * c / smallpi;
+        }
+        in.reconstruct = 1;
+      }
+    }
+  }
+  const int n = static_cast<int>(js.size());
+  AH_HIP(hipMemcpyAsync(buf.d_sin, buf.h_sin, sizeof(SpecIn) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_ah_spectral, dim3(n), dim3(kSpecBlock), 0, st, buf.d_sin, buf.d_sout);
+  AH_HIP(hipGetLastError());
+  AH_HIP(hipMemcpyAsync(buf.h_sout, buf.d_sout, sizeof(SpecOut) * n, hipMemcpyDeviceToHost, st));
+  AH_HIP(hipStreamSynchronize(st));
+  for (size_t q = 0; q < js.size(); ++q) {
+    Job& j = *js[q];
+    AhState& a = *j.h->ah;
+    const oa_params& p = j.h->p;
+    SpecOut& o = buf.h_sout[q];
+    if (o.ac[0] < p.dr0 + p.dz0) o.err = 1;                                 // :1700-1702
+    const int found = (o.iter < p.ahmaxiter && !o.err) ? 1 : 0;
+    a.found[j.k] = found;
+    if (found)
+      for (int m = 0; m <= kSpecModes; ++m) a.ack[j.k][m] = o.ac[m];
+    a.first_sp[j.k] = 0;                                                     // :1748-1754
+    a.shape_spec[j.k] = 1;
+    a.spec_dtheta[j.k] = buf.h_sin[q].dtheta;
+    a.n_of[j.k] = buf.h_sin[q].ntheta + 1;
+    j.spec = o;
+  }
+  return OA_OK;
+}
+
+// Second part: the searches of `jobs` on stream st and their results.
+int search_run(std::vector<Job>& jobs, hipStream_t st, oa_horizon* out) {
+  // shooting searches that precede the flow (ahmethod shoot and both)
+  AH_RC(run_walks(jobs, st));
+  std::vector<Job*> proj, flows;
+  for (Job& j : jobs)
+    if (j.flow && j.shoot_first && j.h->ah->found[j.k] && j.shots > 0) proj.push_back(&j);
+  AH_RC(project_shots(proj, st));
+  // the flow's own start (:1286-1303): ack cleared on the first call unless a
+  // horizon is known, then the shooting search of ahshootfirst
+  bool again = false;
+  for (Job& j : jobs) {
+    if (!j.flow) continue;
+    AhState& a = *j.h->ah;
+    if (a.first_sp[j.k] && !a.found[j.k])
+      for (int m = 0; m <= kSpecModes; ++m) a.ack[j.k][m] = 0.0;
+    j.done = !j.shoot_in_flow;
+    if (j.shoot_in_flow) {
+      j.s.found_in = a.found[j.k];
+      again = true;
+    }
+  }
+  if (again) {
+    AH_RC(run_walks(jobs, st));
+    proj.clear();
+    for (Job& j : jobs)
+      if (j.shoot_in_flow && j.h->ah->found[j.k] && j.shots > 0) proj.push_back(&j);
+    AH_RC(project_shots(proj, st));
+    for (Job& j : jobs)
+      if (j.shoot_in_flow) j.h->ah->ack[j.k][0] = 1.05 * j.h->ah->ack[j.k][0];
+  }
+  for (Job& j : jobs)
+    if (j.flow) flows.push_back(&j);
+  AH_RC(run_flows(flows, st));
 
   const double smallpi = M_PI, half = 0.5, two = 2.0;
   for (size_t q = 0; q < jobs.size(); ++q) {
     Job& j = jobs[q];
     const Search& s = j.s;
     AhState& a = *j.h->ah;
-    a.found[j.k] = s.found;
     oa_horizon& o = out[q];
     std::memset(&o, 0, sizeof o);
-    o.found = s.found;
-    o.ntheta = s.last.ntheta;
-    o.shots = s.shots;
-    o.errorflag = s.errorflag;
+    o.method = j.h->p.ahmethod;
+    o.found = a.found[j.k];
     o.zcenter = s.zcenter;
-    o.zinit = s.zinit;
-    o.dradlast = s.dradlast;
-    double ah_area = 0.0;                                                    // :989-997
-    if (s.found) {
-      ah_area = s.last.area;
-      if (j.h->p.eqsym && s.zcenter == 0.0) ah_area = two * ah_area;
+    o.shots = j.shots;
+    o.zinit = j.zinit;
+    o.dradlast = j.dradlast;
+    double ah_area = 0.0;
+    if (!j.flow) {
+      o.ntheta = s.last.ntheta;
+      o.errorflag = s.errorflag;
+      if (o.found) ah_area = s.last.area;                                    // :989-997
+    } else {
+      o.ntheta = 10 * j.h->p.ahNmodes;
+      o.errorflag = j.spec.err;
+      o.iterations = j.spec.iter;
+      o.residual = j.spec.res;
+      if (o.found) ah_area = j.spec.area;                                    // :1826-1843
     }
+    if (o.found && j.h->p.eqsym && s.zcenter == 0.0) ah_area = two * ah_area;
     o.area = ah_area;
     o.radius = half * std::sqrt(ah_area / smallpi);
-    o.mass = s.found ? half * o.radius : 0.0;                                // :1065-1076
+    o.mass = o.found ? half * o.radius : 0.0;                                // :1065-1076, :1935-1945
   }
   return OA_OK;
 }
@@ -1231,15 +1683,24 @@ extern "C" int oa_horizon_shape(oa_handle* h, int k, double* theta, double* rad,
     return OA_ERR_STATE;
   }
   AhState& a = *h->ah;
-  if (a.gen_of[k] != a.gen) {
-    oa_set_error("oa_horizon_shape: a later search rebuilt the horizon records; search this horizon again");
-    return OA_ERR_STATE;
-  }
   const int m = a.n_of[k];
   *n = m;
   if (!theta || !rad || cap < m) {
     oa_set_error("oa_horizon_shape: arrays too small (need *n entries)");
     return OA_ERR_ARG;
+  }
+  if (a.shape_spec[k]) {                                                     // ah_radius_SP (:1766-1790)
+    AH_HIP(hipMemcpyAsync(rad, a.d_srad[k], sizeof(double) * m, hipMemcpyDeviceToHost, h->stream));
+    AH_HIP(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < m; ++i) {
+      theta[i] = static_cast<double>(i) * a.spec_dtheta[k];
+      if (!a.found[k]) rad[i] = 0.0;
+    }
+    return OA_OK;
+  }
+  if (a.gen_of[k] != a.gen) {
+    oa_set_error("oa_horizon_shape: a later search rebuilt the horizon records; search this horizon again");
+    return OA_ERR_STATE;
   }
   // the search's last integration again, recording its accepted points
   if (!a.d_theta[k]) {

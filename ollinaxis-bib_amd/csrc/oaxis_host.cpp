@@ -760,7 +760,7 @@ extern "C" int oa_params_footprint(const oa_params* pin, double* bytes) {
                static_cast<double>(fb_points(G, G.g + 2) + fb_points(G, G.g - 1));
     if (p.ahfind) total += ah_record_doubles() * plane;   // horizon finder: point records
   }
-  if (p.ahfind) total += ah_fixed_bytes(p.N_ah < 1 ? 1 : (p.N_ah > 3 ? 3 : p.N_ah));
+  if (p.ahfind) total += ah_fixed_bytes(p.N_ah < 1 ? 1 : (p.N_ah > 3 ? 3 : p.N_ah), p.ahmethod);
   *bytes = total;
   return OA_OK;
 }

@@ -36,6 +36,8 @@ module oaxis_mod
   type, bind(C), public :: oa_horizon
     integer(c_int) :: found, ntheta, shots, errorflag
     real(c_double) :: zcenter, zinit, dradlast, area, radius, mass
+    integer(c_int) :: method, iterations
+    real(c_double) :: residual
   end type oa_horizon
 
   ! storage for one oa_params block

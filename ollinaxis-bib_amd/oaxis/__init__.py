@@ -227,10 +227,12 @@ class Spacetime:
         _check(lib().oa_checkpoint_load(self._h, buf, len(data)), "oa_checkpoint_load")
 
     def ahfind(self, k=0, shape=False):
-        """horizon_finder (shooting) for horizon k on the current state:
-        dict(found, ntheta, shots, errorflag, zcenter, zinit, dradlast, area,
-        radius, mass); with shape=True also theta and rad of the last
-        integration (numpy arrays)."""
+        """horizon_finder for horizon k on the current state with the
+        parameters' ahmethod (shoot, spectral or both): dict(found, ntheta,
+        shots, errorflag, zcenter, zinit, dradlast, area, radius, mass,
+        method, iterations, residual); with shape=True also theta and rad
+        of the last integration (shoot) or of the fast flow's last
+        iteration (spectral, both), numpy arrays."""
         res = _abi.Horizon()
         _check(lib().oa_ahfind(self._h, int(k), ctypes.byref(res)), "oa_ahfind")
         out = {name: getattr(res, name) for name, _ in _abi.Horizon._fields_}

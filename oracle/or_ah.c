@@ -1,20 +1,26 @@
 /*
  * ORACLE -- TEST INFRASTRUCTURE ONLY (see or_internal.h).
  *
- * The apparent-horizon finder with the shooting method, restated serially
- * in the reference's own order: horizon_finder (src/geometry/
- * horizon_finder.f90:2-237), ahshoot (:246-1103) and ah_source
- * (:1983-2752), for box 0 with ahmove = .false., ahmethod = shoot and no
- * angular momentum (angmom = .false.: no C1/C2 arrays in this hot path).
- * Interpolation is or_interp (interp.f90:2-270) on the auxiliary arrays of
- * the last auxiliary call.
+ * The apparent-horizon finder restated serially in the reference's own
+ * order: horizon_finder (src/geometry/horizon_finder.f90:2-237) with its
+ * moving centres, ahshoot (:246-1103), ahspectral (:1113-1972) and
+ * ah_source (:1983-2752), for box 0 and no angular momentum (angmom =
+ * .false.: no C1/C2 arrays in this hot path).  Interpolation is or_interp
+ * (interp.f90:2-270) on the auxiliary arrays of the last auxiliary call.
  *
  * Pinned by tests/test_oracle_horizon.py against closed forms: the
  * Schwarzschild horizon at isotropic radius M/2 (mass M), the same horizon
  * seen through a non-trivial axisymmetric coordinate change (A, B, C, H and
  * their derivatives all non-zero), the Painleve-Gullstrand slice whose
  * horizon at r = 2M is fixed by the extrinsic curvature alone, equatorial
- * symmetry, off-centre horizons, and flat space (no horizon).
+ * symmetry, off-centre horizons, and flat space (no horizon).  The spectral
+ * finder by tests/test_oracle_spectral.py: the same closed forms (its fixed
+ * point is the horizon whatever the flow's step control), its cosine
+ * coefficients of a round sphere (only a_0 = 2 r), the agreement of its
+ * surface with the shooting method's on a distorted horizon, and the
+ * projection of a known surface onto coefficients.  Not pinned beyond
+ * that: the step control of the flow (AA halved / grown), which decides the
+ * iteration count only.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -510,15 +516,227 @@ static void ah_shoot(const or_handle* H, const ah_curv* K, double zcenter, int* 
   out->mass = *found ? half * out->radius : zero;         /* :1065-1076 */
 }
 
+#define AH_MODES 50          /* ac(0:50), ack(0:2,0:50) (:1165-1168) */
+
+/* ahshoot's projection of the surface of its last integration onto cosine
+ * coefficients (:872-910), kept in ack(k,:) for the spectral finder:
+ * trapezoids over the integration's uneven points, modes 0..20 (even ones
+ * only with equatorial symmetry about a centre at 0); every other
+ * coefficient is cleared. */
+static void ah_shoot_coefficients(const oa_params* p, double zcenter, int n, const double* theta,
+                                  const double* rad, double* ack) {
+  const double half = 0.5, two = 2.0;
+  const double smallpi = acos(-1.0);
+  const int iaux = (p->eqsym && zcenter == 0.0) ? 2 : 1;
+  for (int j = 0; j <= AH_MODES; ++j) ack[j] = 0.0;
+  for (int j = 0; j <= 20; j += iaux) {
+    for (int i = 0; i < n; ++i)
+      ack[j] = ack[j] + half * (theta[i + 1] - theta[i]) *
+                            (rad[i + 1] * cos((double)j * theta[i + 1]) + rad[i] * cos((double)j * theta[i]));
+    ack[j] = two * (double)iaux * ack[j] / smallpi;
+  }
+}
+
+/* h(theta_i) and h'(theta_i) from the cosine coefficients (:1320-1327,
+ * :1602-1609) */
+static void ah_reconstruct(int N, int Nmode, int step, const double* theta, const double* ac, double* rad,
+                           double* drad) {
+  for (int i = 0; i <= N; ++i) {
+    double r = 0.5 * ac[0], d = 0.0;
+    for (int j = step; j <= Nmode; j += step) {
+      r = r + ac[j] * cos((double)j * theta[i]);
+      d = d - (double)j * ac[j] * sin((double)j * theta[i]);
+    }
+    rad[i] = r;
+    drad[i] = d;
+  }
+}
+
+/* ahspectral (:1113-1972) for horizon k centred at zcenter: the fast flow
+ * of the cosine coefficients of h(theta) (Gundlach, gr-qc/9707050), two
+ * Runge-Kutta stages per iteration, with the step AA halved when the two
+ * stages and one Euler step disagree and grown again (up to ahAA) when they
+ * agree closely.  *found is ahfound(k) and ack its coefficients ack(k,:),
+ * both shared with ahshoot; *firstcall is the finder's own first-call flag
+ * of this horizon.  The surface of the last iteration goes to theta_out /
+ * rad_out (Ntheta + 1 points), which ahshootfirst's search also uses as its
+ * scratch. */
+static void ah_spectral(const or_handle* H, const ah_curv* K, double zcenter, int* found, double* ack,
+                        int* firstcall, oa_horizon* out, double* theta_out, double* rad_out) {
+  const oa_params* p = &H->p;
+  const or_level* L0 = &H->L[0];
+  const double zero = 0.0, one = 1.0, two = 2.0, half = 0.5;
+  const double smallpi = acos(-1.0);
+  const int maxiter = p->ahmaxiter;
+  const int Nmode = p->ahNmodes;                          /* :1221-1232 */
+  const int N = 10 * Nmode;
+  const int step = (p->eqsym && zcenter == 0.0) ? 2 : 1;  /* :1250-1258 */
+  const double dtheta = smallpi / (double)(step * N);
+  const double ahrmax = (p->ahrmax > 0.9) ? 0.9 : p->ahrmax;   /* :1273-1280 */
+  double AA = p->ahAA;
+  const double BB = p->ahBB;
+  double ac[AH_MODES + 1], cc[AH_MODES + 1], ac_old[AH_MODES + 1], cc_old[AH_MODES + 1];
+  double* theta = malloc(sizeof(double) * (size_t)(N + 1));
+  double* rad = malloc(sizeof(double) * (size_t)(N + 1));
+  double* drad = malloc(sizeof(double) * (size_t)(N + 1));
+  double* rad_p = malloc(sizeof(double) * (size_t)(N + 1));
+  double* drad_p = malloc(sizeof(double) * (size_t)(N + 1));
+  double* Ifunc = malloc(sizeof(double) * (size_t)(N + 1));
+  double* area = malloc(sizeof(double) * (size_t)(N + 1));
+  for (int j = 0; j <= AH_MODES; ++j) ac[j] = cc[j] = ac_old[j] = cc_old[j] = zero;
+  for (int i = 0; i <= N; ++i) {
+    theta[i] = (double)i * dtheta;
+    area[i] = zero;
+    Ifunc[i] = zero;
+  }
+  memset(out, 0, sizeof *out);
+
+  if (*firstcall && !*found)                              /* :1286-1289 */
+    for (int j = 0; j <= AH_MODES; ++j) ack[j] = zero;
+  if (*firstcall && p->ahshootfirst) {                    /* :1292-1303 */
+    ah_shoot(H, K, zcenter, found, out, theta_out, rad_out);
+    if (*found) ah_shoot_coefficients(p, zcenter, out->ntheta, theta_out, rad_out, ack);
+    ack[0] = 1.05 * ack[0];
+  }
+  const int shots = out->shots;
+  const double zinit = out->zinit, dradlast = out->dradlast;
+
+  if (*found) {                                           /* :1311-1330 */
+    for (int j = 0; j <= AH_MODES; ++j) ac[j] = ack[j];
+    ah_reconstruct(N, Nmode, step, theta, ac, rad, drad);
+  } else {                                                /* :1340-1356 */
+    if (zcenter == 0.0) ac[0] = (p->ahzinit == 0.0) ? 1.8 * ahrmax * L0->zmax : 2.0 * p->ahzinit;
+    else ac[0] = 1.95 * fabs(zcenter);
+    for (int i = 0; i <= N; ++i) {
+      rad[i] = half * ac[0];
+      drad[i] = zero;
+    }
+    if (zcenter != 0.0 && p->aheccentric != 0.0) {        /* :1365-1407 */
+      for (int i = 0; i <= N; ++i) {
+        const double e = p->aheccentric * sin(theta[i]);
+        rad[i] = half * ac[0] / sqrt(one - e * e);
+      }
+      for (int j = 0; j <= Nmode; ++j) {
+        double a = half * dtheta * rad[0];
+        for (int i = 1; i < N; ++i) a = a + dtheta * rad[i] * cos((double)j * theta[i]);
+        a = a + half * dtheta * rad[N] * cos((double)j * theta[N]);
+        ac[j] = two * (double)step * a / smallpi;
+      }
+      ah_reconstruct(N, Nmode, step, theta, ac, rad, drad);
+    }
+  }
+
+  double res = 1.0;                                       /* :1448-1451 */
+  int iter = 0, errorflag = 0;
+  while (res > p->ahepsilon_spectral && iter < maxiter) {
+    ++iter;
+    for (int j = 0; j <= AH_MODES; ++j) {
+      ac_old[j] = ac[j];
+      cc_old[j] = cc[j];
+    }
+    memcpy(rad_p, rad, sizeof(double) * (size_t)(N + 1));
+    memcpy(drad_p, drad, sizeof(double) * (size_t)(N + 1));
+    for (int q = 1; q <= 2; ++q) {
+      for (int i = 0; i <= N; ++i) {                      /* :1494-1501 */
+        double sdh, sarea;
+        errorflag = ah_source(H, K, rad[i], drad[i], theta[i], zcenter, &sdh, &sarea);
+        if (errorflag) goto found_check;
+        Ifunc[i] = sdh;
+        area[i] = sarea;
+      }
+      Ifunc[0] = Ifunc[1];                                /* :1506-1515 */
+      area[0] = zero;
+      if (step == 1) {
+        Ifunc[N] = Ifunc[N - 1];
+        area[N] = zero;
+      }
+      for (int j = 0; j <= Nmode; j += step) {            /* :1539-1563 */
+        double c = half * dtheta * Ifunc[0];
+        for (int i = 1; i < N; ++i) c = c + dtheta * Ifunc[i] * cos((double)j * theta[i]);
+        c = c + half * dtheta * Ifunc[N] * cos((double)j * theta[N]);
+        c = two * (double)step * c / smallpi;
+        c = c + ((double)j * (double)j) * ac[j];
+        cc[j] = c;
+        if (q == 1) cc_old[j] = c;
+      }
+      for (int j = 0; j <= Nmode; j += step)              /* :1590-1592 */
+        ac[j] = ac_old[j] - half * AA * (double)q / (one + BB * ((double)j * (double)j)) * cc[j];
+      ah_reconstruct(N, Nmode, step, theta, ac, rad, drad);
+    }
+    res = zero;                                           /* :1636-1640 */
+    for (int j = 0; j <= Nmode; j += step) res = fmax(res, fabs(cc[j]));
+    double aux1 = zero, aux2 = zero;                      /* :1651-1682 */
+    for (int j = 0; j <= Nmode; j += step) {
+      const double tc = ac_old[j] - AA / (one + BB * ((double)j * (double)j)) * cc_old[j];
+      aux1 = aux1 + fabs(ac[j] - tc);
+      aux2 = aux2 + fabs(ac_old[j]);
+    }
+    if (aux1 > 0.05 * aux2) {
+      AA = half * AA;
+      for (int j = 0; j <= Nmode; j += step) ac[j] = ac_old[j];
+      memcpy(rad, rad_p, sizeof(double) * (size_t)(N + 1));
+      memcpy(drad, drad_p, sizeof(double) * (size_t)(N + 1));
+    } else if (aux1 < 0.001 * aux2) {
+      AA = fmin(1.5 * AA, p->ahAA);
+    }
+  }
+
+found_check:                                              /* :1700-1733 */
+  if (ac[0] < p->dr0 + p->dz0) errorflag = 1;
+  if (iter < maxiter && !errorflag) {
+    *found = 1;
+    for (int j = 0; j <= AH_MODES; ++j) ack[j] = ac[j];
+  } else {
+    *found = 0;
+  }
+  *firstcall = 0;                                         /* :1748-1754 */
+
+  double ah_area = zero;                                  /* :1826-1843 */
+  if (*found) {
+    ah_area = half * dtheta * area[0];
+    for (int i = 1; i < N; ++i) ah_area = ah_area + dtheta * area[i];
+    ah_area = ah_area + half * dtheta * area[N];
+    if (p->eqsym && zcenter == 0.0) ah_area = two * ah_area;
+  }
+  out->found = *found;
+  out->ntheta = N;
+  out->shots = shots;
+  out->errorflag = errorflag;
+  out->zcenter = zcenter;
+  out->zinit = zinit;
+  out->dradlast = dradlast;
+  out->area = ah_area;
+  out->radius = half * sqrt(ah_area / smallpi);
+  out->mass = *found ? half * out->radius : zero;         /* :1935-1945 */
+  out->method = p->ahmethod;
+  out->iterations = iter;
+  out->residual = res;
+  if (theta_out)
+    for (int i = 0; i <= N; ++i) {                        /* ah_radius_SP (:1766-1790) */
+      theta_out[i] = theta[i];
+      rad_out[i] = *found ? rad[i] : zero;
+    }
+  free(theta);
+  free(rad);
+  free(drad);
+  free(rad_p);
+  free(drad_p);
+  free(Ifunc);
+  free(area);
+}
+
 /* horizon_finder (:2-237) for horizon k: sanity checks, the centre, then
- * ahshoot.  KA..KH are formed here from the auxiliary arrays.  Returns 0 or
- * an OA_ERR_* code. */
-int or_ahfind(or_handle* H, int k, int* found, oa_horizon* out, double* theta_out, double* rad_out) {
+ * the method's search (:209-227).  *found and ack (AH_MODES + 1 values)
+ * are ahfound(k) and ack(k,:), *firstcall the spectral finder's first-call
+ * flag; all three carry over between calls.  KA..KH are formed here from
+ * the auxiliary arrays.  Returns 0 or an OA_ERR_* code. */
+int or_ahfind(or_handle* H, int k, int* found, double* ack, int* firstcall, oa_horizon* out, double* theta_out,
+              double* rad_out) {
   const oa_params* p = &H->p;
   if (p->N_ah > 3 || p->N_ah < 1 || k < 0 || k >= p->N_ah) return OA_ERR_ARG;   /* :48-57 */
   if (p->eqsym && p->N_ah == 3) return OA_ERR_PARAM;                              /* :62-71 */
   if (p->z_ah1 < 0.0 || p->z_ah2 > 0.0) return OA_ERR_PARAM;                      /* :76-95 */
-  if (p->ahmethod != OA_AH_SHOOT) return OA_ERR_UNSUPPORTED;
+  if (p->ahmethod != OA_AH_SHOOT && (p->ahNmodes < 1 || p->ahNmodes > AH_MODES)) return OA_ERR_PARAM;
   double zah[3] = {0.0, 0.0, 0.0};                                                 /* :104-111 */
   if (p->N_ah > 1) {
     zah[1] = p->z_ah1;
@@ -549,9 +767,31 @@ int or_ahfind(or_handle* H, int k, int* found, oa_horizon* out, double* theta_ou
       }
     }
   }
+  double *th_scratch = NULL, *rad_scratch = NULL;
+  if (!theta_out || !rad_out) {       /* the spectral paths keep the shooting surface here */
+    th_scratch = malloc(sizeof(double) * (AH_NTHETAMAX + 1));
+    rad_scratch = malloc(sizeof(double) * (AH_NTHETAMAX + 1));
+    theta_out = th_scratch;
+    rad_out = rad_scratch;
+  }
   ah_curv* K = ah_curv_new(H);
-  ah_shoot(H, K, zah[k], found, out, theta_out, rad_out);
+  if (p->ahmethod == OA_AH_SHOOT) {
+    ah_shoot(H, K, zah[k], found, out, theta_out, rad_out);
+  } else if (p->ahmethod == OA_AH_SPECTRAL) {
+    ah_spectral(H, K, zah[k], found, ack, firstcall, out, theta_out, rad_out);
+  } else {
+    oa_horizon shot;
+    memset(&shot, 0, sizeof shot);
+    ah_shoot(H, K, zah[k], found, &shot, theta_out, rad_out);
+    if (*found) ah_shoot_coefficients(p, zah[k], shot.ntheta, theta_out, rad_out, ack);
+    ah_spectral(H, K, zah[k], found, ack, firstcall, out, theta_out, rad_out);
+    out->shots = shot.shots;
+    out->zinit = shot.zinit;
+    out->dradlast = shot.dradlast;
+  }
   ah_curv_free(H, K);
+  free(th_scratch);
+  free(rad_scratch);
   return 0;
 }
 
@@ -567,12 +807,14 @@ int or_ah_integrate(or_handle* H, double zinit, double zcenter, double* res) {
   return 0;
 }
 
-/* or_ahfind with the result as ten doubles (found, ntheta, shots, errorflag,
- * zcenter, zinit, dradlast, area, radius, mass), for the ctypes tests. */
-int or_ahfind_flat(or_handle* H, int k, int* found, double* res, double* theta_out, double* rad_out) {
+/* or_ahfind with the result as thirteen doubles (found, ntheta, shots,
+ * errorflag, zcenter, zinit, dradlast, area, radius, mass, method,
+ * iterations, residual), for the ctypes tests. */
+int or_ahfind_flat(or_handle* H, int k, int* found, double* ack, int* firstcall, double* res, double* theta_out,
+                   double* rad_out) {
   oa_horizon o;
   memset(&o, 0, sizeof o);
-  const int rc = or_ahfind(H, k, found, &o, theta_out, rad_out);
+  const int rc = or_ahfind(H, k, found, ack, firstcall, &o, theta_out, rad_out);
   res[0] = o.found;
   res[1] = o.ntheta;
   res[2] = o.shots;
@@ -583,5 +825,8 @@ int or_ahfind_flat(or_handle* H, int k, int* found, double* res, double* theta_o
   res[7] = o.area;
   res[8] = o.radius;
   res[9] = o.mass;
+  res[10] = o.method;
+  res[11] = o.iterations;
+  res[12] = o.residual;
   return rc;
 }

@@ -56,7 +56,7 @@ def lib():
             "or_run_restrict": (ctypes.c_int, [H, ctypes.c_int]),
             "or_run_update": (ctypes.c_int, [H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             "or_run_symmetries": (ctypes.c_int, [H, ctypes.c_int]),
-            "or_ahfind_flat": (ctypes.c_int, [H, ctypes.c_int, ip, dp, dp, dp]),
+            "or_ahfind_flat": (ctypes.c_int, [H, ctypes.c_int, ip, dp, ip, dp, dp, dp]),
             "or_ah_integrate": (ctypes.c_int, [H, ctypes.c_double, ctypes.c_double, dp]),
             "or_ah_set_tolerance": (None, [ctypes.c_double, ctypes.c_double]),
             "or_ell_stats": (None, [H, ip, ip, dp]),
@@ -214,30 +214,43 @@ class Oracle:
     def run_restrict(self, level):
         lib().or_run_restrict(self._h, level)
 
-    AH_KEYS = ("found", "ntheta", "shots", "errorflag", "zcenter", "zinit", "dradlast", "area", "radius", "mass")
+    AH_KEYS = ("found", "ntheta", "shots", "errorflag", "zcenter", "zinit", "dradlast", "area", "radius", "mass",
+               "method", "iterations", "residual")
 
     def ahfind(self, k=0, shape=False):
         """horizon_finder for horizon k on the current state and auxiliary
-        arrays (every level).  ahfound(k) carries over between calls as in
-        the reference.  Returns a dict; with shape=True also theta and rad of
-        the last integration."""
+        arrays (every level).  ahfound(k), the cosine coefficients ack(k,:)
+        and the spectral finder's first-call flag carry over between calls
+        as in the reference.  Returns a dict; with shape=True also theta and
+        rad of the last integration (shooting) or of the last fast-flow
+        iteration (spectral, both)."""
         if not hasattr(self, "_ahfound"):
             self._ahfound = [ctypes.c_int(0) for _ in range(3)]
-        res = np.zeros(10)
+            self._ahfirst = [ctypes.c_int(1) for _ in range(3)]
+            self._ahack = np.zeros((3, 51))
+        res = np.zeros(13)
         th = np.zeros(50001)
         rd = np.zeros(50001)
-        rc = lib().or_ahfind_flat(self._h, int(k), ctypes.byref(self._ahfound[k]), _dptr(res), _dptr(th), _dptr(rd))
+        ack = self._ahack[k]
+        rc = lib().or_ahfind_flat(self._h, int(k), ctypes.byref(self._ahfound[k]), _dptr(ack),
+                                  ctypes.byref(self._ahfirst[k]), _dptr(res), _dptr(th), _dptr(rd))
         if rc:
             raise RuntimeError("oracle ahfind: error %d" % rc)
         out = dict(zip(self.AH_KEYS, res.tolist()))
-        for key in ("found", "ntheta", "shots", "errorflag"):
+        for key in ("found", "ntheta", "shots", "errorflag", "method", "iterations"):
             out[key] = int(out[key])
         if shape:
             n = out["ntheta"] + 1
-            # ah_radius_SH holds h = 0 when no horizon was found (:962-966)
+            # ah_radius_SH / _SP hold h = 0 when no horizon was found (:962-966, :1783-1787)
             out["theta"] = th[:n].copy()
             out["rad"] = rd[:n].copy() if out["found"] else np.zeros(n)
         return out
+
+    def ah_coefficients(self, k=0):
+        """ack(k,:), the cosine coefficients kept for horizon k (51 values)."""
+        if not hasattr(self, "_ahack"):
+            return np.zeros(51)
+        return self._ahack[k].copy()
 
     def ah_integrate(self, zinit, zcenter=0.0):
         """One integration of the horizon ODE from the axis point zinit:
