@@ -1,9 +1,11 @@
-// Apparent-horizon finder with the shooting method on the device.
+// Apparent-horizon finder on the device: the shooting and the spectral
+// method.
 //
 // Reference: src/geometry/horizon_finder.f90 -- horizon_finder (:2-237),
-// ahshoot (:246-1103), ah_source (:1983-2752) -- with ahmethod = shoot,
-// ahmove = .false. and no angular momentum (no C1/C2 arrays in this hot
-// path), for box 0.
+// ahshoot (:246-1103), ahspectral (:1113-1972), ah_source (:1983-2752) --
+// with no angular momentum (no C1/C2 arrays in this hot path), for box 0.
+// The spectral method is described beside k_ah_spectral; the rest of this
+// note is about shooting.
 //
 // The reference integrates the horizon ODE h'' = sdh(h, h', theta) once per
 // "shot" from an axis point zinit, marching zinit down the axis by half a
