@@ -310,6 +310,34 @@ int oa_norms0d_noalloc(oa_handle* h, int box, int level, int n, const char* cons
 int oa_checkpoint_size(const oa_handle* h, size_t* bytes);
 int oa_checkpoint_save(oa_handle* h, void* buf, size_t bytes);
 int oa_checkpoint_load(oa_handle* h, const void* buf, size_t bytes);
+/* checkpointrestart (checkpoint.f90:338-690), reached from initial.f90:157-161
+ * with idata = checkpoint: read the state from the directory dir (the par
+ * file's checkpointfile) that a single-process reference run wrote with
+ * checkpointsave.  checkparam must agree with this handle's eqsym, angmom,
+ * dr0, dz0, Nrtotal, Nztotal, Nb, Nl0..Nl3 and mattertype (the reference's
+ * checks and messages, :400-530); checklist names the arrays.  For every
+ * level the arrays this spacetime evolves are read from their .2D dumps
+ * (9 significant digits, as the reference wrote them) and their rR, zL, zR
+ * fine-boundary time levels from the _bound_ files, whose headers set the
+ * level clocks t, t1, t2; the step counters restart at 0.  Arrays the
+ * reference also lists but this path does not form (dtalpha, KTH, lambda,
+ * hdet, ...) are skipped.  Then call oa_initial_aux, as evolve.f90:128-195
+ * runs symmetries, the auxiliary pass and the Gamma-driver initialisation
+ * after initial.  oa_initial_data refuses idata = checkpoint
+ * (OA_ERR_STATE) and points here. */
+int oa_checkpoint_restart(oa_handle* h, const char* dir);
+/* checkpointsave (checkpoint.f90:2-328) for one process into dir (created
+ * if missing; the reference names it <directory>/checkpoint_t=<t>):
+ * checkparam, checklist (parfile's name, then r, z, rr and the arrays this
+ * spacetime evolves), each listed array's .2D dump as save2Dvariable writes
+ * it there (9 significant digits, + 1e-30) and, but for the coordinates,
+ * its _bound_rL/rR/zL/zR files with the level clocks (16 digits; rL holds
+ * zeros, box 0 never fills it).  oa_checkpoint_restart reads it back.  The
+ * reference's restart reads the arrays its checklist names, so from this
+ * directory it would start the arrays this path does not form from its own
+ * initial values; without a reference build that is not checked here.
+ * Between coarse steps only. */
+int oa_checkpoint_write(oa_handle* h, const char* dir, const char* parfile);
 
 /* One apparent-horizon search: what ahshoot (src/geometry/horizon_finder.f90:
  * 246-1103) finds for horizon k and writes to its ah_area / ah_mass files. */

@@ -238,8 +238,6 @@ extern "C" int oa_initial_data(oa_handle* h) {
   using namespace oax;
   int nlev = 0;
   if (!h || oa_num_levels(h, &nlev) != OA_OK) return OA_ERR_ARG;
-  int rc = fill_levels(h, nlev, nullptr);
-  if (rc != OA_OK) return rc;
   const oa_params* pp = nullptr;
   {
     double* slab = nullptr;
@@ -248,6 +246,13 @@ extern "C" int oa_initial_data(oa_handle* h) {
     bool shift = false, scalar = false;
     handle_level_slab(h, 0, &slab, &G, &nf, &shift, &scalar, &pp);
   }
+  // initial.f90:157-161: the state comes from the checkpoint directory
+  if (pp->idata == OA_IDATA_CHECKPOINT) {
+    oa_set_error("oa_initial_data: idata = checkpoint; read the state with oa_checkpoint_restart(h, checkpointfile)");
+    return OA_ERR_STATE;
+  }
+  int rc = fill_levels(h, nlev, nullptr);
+  if (rc != OA_OK) return rc;
   if (pp->idata != OA_IDATA_BRILLWAVE && pp->idata != OA_IDATA_SCALARPULSE) return OA_OK;
   // Brill waves and the scalar pulse: solve the Hamiltonian constraint for
   // psi on every level (idata_brillwave.f90:117-135,

@@ -6,9 +6,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "oa_internal.h"
 #include "oaxis.h"
@@ -900,6 +904,329 @@ extern "C" int oa_checkpoint_load(oa_handle* h, const void* buf, size_t bytes) {
     L.cons_valid = false;
   }
   HIPCHK(hipStreamSynchronize(h->stream));
+  return OA_OK;
+}
+
+// ------------------------------------------------------------ reference checkpoint
+// checkpointrestart (src/base/checkpoint.f90:338-690): the state a
+// single-process reference run wrote with checkpointsave (:2-328) into a
+// directory -- checkparam (grid and symmetry settings), checklist (the par
+// file's name, then one array name per line), per array and level the
+// <name>_b0l<level>.2D dump (save2Dvariable: "#Time = t", then r, z, value
+// for j = 1-g..Nz, i = 1-g..Nr, 9 significant digits) and four strip files
+// <name>_bound_{rL,rR,zL,zR}_b0l<level> ("t t1 t2", a blank line, then the
+// three time levels of every strip point, 16 digits).  The arrays this
+// spacetime evolves are read; the others the reference lists (dtalpha, KTH,
+// lambda, hdet, ...) are not formed on this path and are skipped, as are
+// the rL strips (box 0 touches the axis).  The clocks come from the strip
+// headers and the step counters restart at 0, as the reference's do.
+namespace {
+
+bool read_values(std::FILE* f, double* v, int n) {
+  for (int k = 0; k < n; ++k)
+    if (std::fscanf(f, "%lf", &v[k]) != 1) return false;   // list-directed: line breaks anywhere
+  return true;
+}
+
+struct TextFile {
+  std::FILE* f = nullptr;
+  explicit TextFile(const std::string& path) : f(std::fopen(path.c_str(), "r")) {}
+  ~TextFile() {
+    if (f) std::fclose(f);
+  }
+};
+
+int ckref_fail(const std::string& what) {
+  oa_set_error("oa_checkpoint_restart: " + what);
+  return OA_ERR_ARG;
+}
+
+// checkparam: the settings a restart must share (:400-530)
+int ckref_params(const oa_handle* h, const std::string& dir) {
+  TextFile in(dir + "/checkparam");
+  if (!in.f) return ckref_fail("checkpoint directory \"" + dir + "\" has no checkparam");
+  oa_params q;
+  oa_params_default(&q);
+  int nproctot = 1;
+  char line[1024];
+  while (std::fgets(line, sizeof line, in.f)) {
+    std::string l(line);
+    const size_t eq = l.find('=');
+    if (eq == std::string::npos) continue;
+    auto trim = [](std::string x) {
+      const size_t a = x.find_first_not_of(" \t\r\n"), b = x.find_last_not_of(" \t\r\n");
+      return a == std::string::npos ? std::string() : x.substr(a, b - a + 1);
+    };
+    const std::string key = trim(l.substr(0, eq)), val = trim(l.substr(eq + 1));
+    if (key == "nproctot") nproctot = std::atoi(val.c_str());
+    else if (oa_params_set(&q, key.c_str(), val.c_str()) != OA_OK)
+      return ckref_fail("checkparam: cannot read " + key + " = " + val);
+  }
+  const oa_params& p = h->p;
+  if (nproctot != 1)
+    return ckref_fail("the checkpoint was written by " + std::to_string(nproctot) +
+                      " processes; only single-process checkpoints are read");
+  if ((q.eqsym != 0) != (p.eqsym != 0))
+    return ckref_fail("When restarting from a checkpoint the equatorial symmetry flag must be the same.");
+  if ((q.angmom != 0) != (p.angmom != 0))
+    return ckref_fail("When restarting from a checkpoint the angular momentum flag must be the same.");
+  if (q.dr0 != p.dr0 || q.dz0 != p.dz0)
+    return ckref_fail("When restarting from a checkpoint the grid spacing must be the same.");
+  if (q.Nrtotal != p.Nrtotal || q.Nztotal != p.Nztotal)
+    return ckref_fail("When restarting from a checkpoint the grid size must be the same.");
+  if (q.Nb != p.Nb) return ckref_fail("When restarting from a checkpoint the number of boxes must be the same.");
+  if (q.Nl0 != p.Nl0 || q.Nl1 != p.Nl1 || q.Nl2 != p.Nl2 || q.Nl3 != p.Nl3)
+    return ckref_fail("When restarting from a checkpoint the number of levels must be the same.");
+  if (q.mattertype != p.mattertype)
+    return ckref_fail("When restarting from a checkpoint the type of matter must be the same.");
+  return OA_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Fortran ESw.d of one value, right-justified in w columns (the exponent
+// loses its E above 99, as the edit descriptor without Ee writes it)
+std::string fortran_es(double x, int w, int d) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*E", d, x);
+  std::string s(buf);
+  const size_t e = s.find('E');
+  std::string mant = s.substr(0, e);
+  const int ev = std::atoi(s.c_str() + e + 1);
+  char ex[16];
+  if (ev >= -99 && ev <= 99) std::snprintf(ex, sizeof ex, "E%c%02d", ev < 0 ? '-' : '+', ev < 0 ? -ev : ev);
+  else std::snprintf(ex, sizeof ex, "%c%03d", ev < 0 ? '-' : '+', ev < 0 ? -ev : ev);
+  std::string out = mant + ex;
+  if (static_cast<int>(out.size()) > w) return std::string(static_cast<size_t>(w), '*');
+  return std::string(static_cast<size_t>(w) - out.size(), ' ') + out;
+}
+
+// list-directed output of one "key = value" line of checkparam (:180-205)
+std::string ld_int(const char* key, int v) {
+  char b[128];
+  std::snprintf(b, sizeof b, " %s = %12d\n", key, v);
+  return b;
+}
+std::string ld_real(const char* key, double v) {
+  return std::string(" ") + key + " = " + fortran_es(v, 25, 16) + "\n";
+}
+std::string ld_logical(const char* key, int v) { return std::string(" ") + key + " =  " + (v ? "T" : "F") + "\n"; }
+
+int ckref_write(const std::string& path, const std::string& text) {
+  std::FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return ckref_fail("cannot write " + path);
+  const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+  std::fclose(f);
+  return ok ? OA_OK : ckref_fail("cannot write " + path);
+}
+
+}  // namespace
+
+// checkpointsave (checkpoint.f90:2-328) for a single process into the
+// directory dir (made if missing; the reference names it
+// <directory>/checkpoint_t=<t>): checkparam, checklist (parfile, then r, z,
+// rr and the arrays this spacetime evolves), every listed array's .2D dump
+// (save2Dvariable with status replace: 9 significant digits, + 1e-30) and,
+// except for the coordinates, its four _bound_ strip files with the level's
+// clocks (16 digits).  The rL strips, which box 0 never fills, hold zeros.
+extern "C" int oa_checkpoint_write(oa_handle* h, const char* dir, const char* parfile) {
+  if (!h || !dir) return OA_ERR_ARG;
+  RC(ckpt_check(h, "oa_checkpoint_write"));
+  const std::string d(dir);
+  for (size_t k = 1; k <= d.size(); ++k)                 // mkdir -p
+    if (k == d.size() || d[k] == '/') (void)::mkdir(d.substr(0, k).c_str(), 0755);
+  const oa_params& p = h->p;
+  std::string cp;
+  cp += ld_int("nproctot", 1);
+  cp += ld_logical("eqsym", p.eqsym);
+  cp += ld_logical("angmom", p.angmom);
+  cp += ld_real("dr0", p.dr0);
+  cp += ld_real("dz0", p.dz0);
+  cp += ld_int("Nrtotal", p.Nrtotal);
+  cp += ld_int("Nztotal", p.Nztotal);
+  cp += ld_int("Nb", p.Nb);
+  cp += ld_int("Nl0", p.Nl0);
+  cp += ld_int("Nl1", p.Nl1);
+  cp += ld_int("Nl2", p.Nl2);
+  cp += ld_int("Nl3", p.Nl3);
+  for (const char* k : {"Nrbox1", "Nzbox1", "Nrbox2", "Nzbox2", "Nrbox3", "Nzbox3"}) cp += ld_int(k, 0);
+  for (const char* k : {"rbox1", "zbox1", "rbox2", "zbox2", "rbox3", "zbox3"}) cp += ld_real(k, 0.0);
+  cp += std::string(" mattertype = ") + ((p.mattertype & OA_MATTER_SCALAR) ? "scalar" : "vacuum") + "\n";
+  RC(ckref_write(d + "/checkparam", cp));
+  static const char* coords[] = {"r", "z", "rr"};
+  std::vector<std::string> names(coords, coords + 3);
+  {
+    static const char* cand[] = {"alpha",    "chi",      "phi",         "A",           "B",           "H",
+                                 "C",        "trK",      "KTA",         "KTB",         "KTC",         "Delta_r",
+                                 "Delta_z",  "beta_r",   "beta_z",      "dtbeta_r",    "dtbeta_z",    "scalar_phi",
+                                 "scalar_xi_r", "scalar_xi_z", "scalar_pi"};
+    std::vector<std::pair<int, std::string>> ev;
+    for (const char* c : cand)
+      if (slot_of(h, c) >= 0) ev.push_back({slot_of(h, c), c});
+    std::sort(ev.begin(), ev.end());
+    for (auto& e : ev) names.push_back(e.second);
+  }
+  std::string cl = std::string(" ") + (parfile ? parfile : "checkpoint.par") + "\n";
+  for (const std::string& n : names) cl += " " + n + "\n";
+  RC(ckref_write(d + "/checklist", cl));
+
+  for (int l = 0; l < h->nlev; ++l) {
+    Level& L = h->L[l];
+    const KGrid& G = L.G;
+    const int g = G.g, nrt = G.Nr + g, nzt = G.Nz + g;
+    const std::string filen = "_b0l" + std::to_string(l);
+    const size_t sset = static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
+    std::vector<double> strips[3];
+    for (int tl = 0; tl < 3; ++tl) {
+      strips[tl].resize(sset);
+      HIPCHK(hipMemcpyAsync(strips[tl].data(), L.strip[L.ring[tl]], sizeof(double) * sset, hipMemcpyDeviceToHost,
+                            h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> plane(static_cast<size_t>(G.plane));
+    for (const std::string& n : names) {
+      const int slot = slot_of(h, n);
+      if (slot >= 0) RC(oa_download_field(h, 0, l, n.c_str(), plane.data(), nrt));   // phi as the reference has it
+      std::string out = "#Time = " + fortran_es(L.t, 23, 15) + "\n";
+      for (int jj = 0; jj < nzt; ++jj) {
+        const double z = (static_cast<double>(jj + 1 - g) - G.zoff) * G.dz;
+        for (int ii = 0; ii < nrt; ++ii) {
+          const double r = (static_cast<double>(ii + 1 - g) - 0.5) * G.dr;
+          double v;
+          if (slot >= 0) v = plane[static_cast<size_t>(jj) * nrt + ii];
+          else if (n == "r") v = r;
+          else if (n == "z") v = z;
+          else v = std::sqrt(r * r + z * z);
+          out += fortran_es(r, 16, 8) + fortran_es(z, 16, 8) + fortran_es(v + 1.0e-30, 16, 8) + "\n";
+        }
+        out += "\n";
+      }
+      out += "\n";
+      RC(ckref_write(d + "/" + n + filen + ".2D", out));
+      if (slot < 0) continue;
+      const std::string head = fortran_es(L.t, 23, 15) + fortran_es(L.t1, 23, 15) + fortran_es(L.t2, 23, 15) + "\n\n";
+      const char* piece_name[4] = {"_bound_rL", "_bound_rR", "_bound_zL", "_bound_zR"};
+      for (int pc = 0; pc < 4; ++pc) {
+        std::string sv = head;
+        const int piece = pc - 1;                      // -1: rL (zeros)
+        const long per = piece <= 0 ? L.strip_r : L.strip_z;
+        const long off = piece <= 0 ? 0 : (piece == 1 ? h->nf * L.strip_r : h->nf * L.strip_r + h->nf * L.strip_z);
+        const int outer = piece <= 0 ? g : nrt, inner = piece <= 0 ? nzt : g;
+        for (int a = 0; a < outer; ++a)
+          for (int b = 0; b < inner; ++b) {
+            const long q = piece <= 0 ? static_cast<long>(a) * nzt + b : static_cast<long>(b) * nrt + a;
+            for (int tl = 0; tl < 3; ++tl)
+              sv += fortran_es(piece < 0 ? 0.0 : strips[tl][static_cast<size_t>(off + slot * per + q)], 23, 15);
+            sv += "\n";
+          }
+        RC(ckref_write(d + "/" + n + piece_name[pc] + filen, sv));
+      }
+    }
+  }
+  return OA_OK;
+}
+
+extern "C" int oa_checkpoint_restart(oa_handle* h, const char* dir) {
+  if (!h || !dir) return OA_ERR_ARG;
+  RC(ckpt_check(h, "oa_checkpoint_restart"));
+  const std::string d(dir);
+  RC(ckref_params(h, d));
+  std::vector<std::string> names;                                  // checklist (:553-565)
+  {
+    TextFile in(d + "/checklist");
+    if (!in.f) return ckref_fail("checkpoint directory \"" + d + "\" has no checklist");
+    char line[1024];
+    bool first = true;
+    while (std::fgets(line, sizeof line, in.f)) {
+      char word[1024];
+      if (std::sscanf(line, "%1023s", word) != 1) continue;
+      if (first) {                                                   // the par file's name
+        first = false;
+        continue;
+      }
+      names.push_back(word);
+    }
+  }
+  std::vector<bool> seen(static_cast<size_t>(h->nf), false);
+  for (const std::string& n : names) {
+    const int slot = slot_of(h, n);
+    if (slot >= 0) seen[static_cast<size_t>(slot)] = true;
+  }
+  for (int sl = 0; sl < h->nf; ++sl)
+    if (!seen[static_cast<size_t>(sl)]) return ckref_fail("the checklist lacks an array this spacetime evolves");
+
+  for (int l = 0; l < h->nlev; ++l) {
+    Level& L = h->L[l];
+    const KGrid& G = L.G;
+    const int g = G.g, nrt = G.Nr + g, nzt = G.Nz + g;
+    const std::string filen = "_b0l" + std::to_string(l);
+    std::vector<double> plane(static_cast<size_t>(G.plane));
+    const size_t sset = static_cast<size_t>(h->nf) * (L.strip_r + 2 * L.strip_z);
+    std::vector<double> strips[3];
+    for (auto& v : strips) v.assign(sset, 0.0);
+    double clk[3] = {L.t, L.t1, L.t2};
+    for (const std::string& n : names) {
+      const int slot = slot_of(h, n);
+      if (slot < 0) continue;
+      {                                                              // read2Ddata (read2Ddata.f90:38-66)
+        const std::string path = d + "/" + n + filen + ".2D";
+        TextFile in(path);
+        if (!in.f) return ckref_fail("cannot open " + path);
+        char line[1024];
+        const long start = std::ftell(in.f);
+        bool header = false;
+        if (std::fgets(line, sizeof line, in.f)) {
+          const char* c = line;
+          while (*c == ' ' || *c == '\t') ++c;
+          header = *c == '#';
+        }
+        if (!header) std::fseek(in.f, start, SEEK_SET);
+        std::fill(plane.begin(), plane.end(), 0.0);
+        for (int jj = 0; jj < nzt; ++jj)
+          for (int ii = 0; ii < nrt; ++ii) {
+            double v[3];
+            if (!read_values(in.f, v, 3)) return ckref_fail(path + ": fewer values than the level holds");
+            plane[static_cast<size_t>(ii) * G.ld + jj] = v[2];
+          }
+        HIPCHK(hipMemcpyAsync(L.slab[L.cur] + static_cast<long>(slot) * G.plane, plane.data(),
+                              sizeof(double) * plane.size(), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+      }
+      // the strips: rR (piece 0), zL (1), zR (2); the file loops of :596-650
+      const char* piece_name[3] = {"_bound_rR", "_bound_zL", "_bound_zR"};
+      for (int piece = 0; piece < 3; ++piece) {
+        const std::string path = d + "/" + n + piece_name[piece] + filen;
+        TextFile in(path);
+        if (!in.f) return ckref_fail("cannot open " + path);
+        if (!read_values(in.f, clk, 3)) return ckref_fail(path + ": no clocks");
+        const long per = piece == 0 ? L.strip_r : L.strip_z;
+        const long off = piece == 0 ? 0 : (piece == 1 ? h->nf * L.strip_r : h->nf * L.strip_r + h->nf * L.strip_z);
+        const int outer = piece == 0 ? g : nrt, inner = piece == 0 ? nzt : g;
+        for (int a = 0; a < outer; ++a)
+          for (int b = 0; b < inner; ++b) {
+            double v[3];
+            if (!read_values(in.f, v, 3)) return ckref_fail(path + ": fewer values than the strip holds");
+            // rR(m, k): m = a, k = b + 1 - g; zL / zR(i, n): i = a + 1 - g, n = b
+            const long q = piece == 0 ? static_cast<long>(a) * nzt + b : static_cast<long>(b) * nrt + a;
+            for (int tl = 0; tl < 3; ++tl) strips[tl][static_cast<size_t>(off + slot * per + q)] = v[tl];
+          }
+      }
+    }
+    for (int tl = 0; tl < 3; ++tl)
+      HIPCHK(hipMemcpyAsync(L.strip[L.ring[tl]], strips[tl].data(), sizeof(double) * sset, hipMemcpyHostToDevice,
+                            h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    L.t = clk[0];
+    L.t1 = clk[1];
+    L.t2 = clk[2];
+    L.s = 0;                                                         // evolve.f90: s = 0 before initial
+    L.dirty = true;
+    L.phi_valid = true;
+    L.cons_valid = false;
+    L.dtb_pre_valid = false;
+  }
   return OA_OK;
 }
 

@@ -30,7 +30,8 @@ module oaxis_mod
             oa_batch_rhs_kernel_stats, oa_set_graphs, oa_batch_set_graphs, oa_schedule, oa_batch_schedule, &
             oa_constraints, oa_ahfind, oa_horizon_shape, oa_batch_ahfind, &
             oa_ahfind_begin, oa_batch_ahfind_begin, oa_ahfind_end, &
-            oa_checkpoint_size, oa_checkpoint_save, oa_checkpoint_load
+            oa_checkpoint_size, oa_checkpoint_save, oa_checkpoint_load, oa_checkpoint_restart, &
+            oa_checkpoint_write
 
   ! result of one apparent-horizon search (oa_horizon in include/oaxis.h)
   type, bind(C), public :: oa_horizon
@@ -264,7 +265,19 @@ module oaxis_mod
       type(c_ptr), value :: h, buf
       integer(c_size_t), value :: bytes
     end function
-    ! horizon_finder (shooting) for horizon k on the current state
+    ! checkpointsave: the state as a reference checkpoint directory
+    integer(c_int) function oa_checkpoint_write(h, dir, parfile) bind(C, name="oa_checkpoint_write")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*) :: dir, parfile
+    end function
+    ! checkpointrestart: the state from a reference checkpoint directory
+    integer(c_int) function oa_checkpoint_restart(h, dir) bind(C, name="oa_checkpoint_restart")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*) :: dir
+    end function
+    ! horizon_finder for horizon k on the current state (par file's ahmethod)
     integer(c_int) function oa_ahfind(h, k, res) bind(C, name="oa_ahfind")
       import :: c_int, c_ptr, oa_horizon
       type(c_ptr), value :: h

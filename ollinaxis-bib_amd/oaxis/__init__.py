@@ -226,6 +226,19 @@ class Spacetime:
         buf = ctypes.create_string_buffer(bytes(data), len(data))
         _check(lib().oa_checkpoint_load(self._h, buf, len(data)), "oa_checkpoint_load")
 
+    def restart(self, directory):
+        """checkpointrestart: read the state from a checkpoint directory in
+        the reference's format (checkparam, checklist, the .2D dumps and the
+        _bound_ strip files of every array and level) for a spacetime
+        created with idata = checkpoint; then call initial_aux()."""
+        _check(lib().oa_checkpoint_restart(self._h, str(directory).encode()), "oa_checkpoint_restart")
+
+    def checkpoint_write(self, directory, parfile="checkpoint.par"):
+        """checkpointsave: write the state as a checkpoint directory in the
+        reference's format (read back by restart())."""
+        _check(lib().oa_checkpoint_write(self._h, str(directory).encode(), str(parfile).encode()),
+               "oa_checkpoint_write")
+
     def ahfind(self, k=0, shape=False):
         """horizon_finder for horizon k on the current state with the
         parameters' ahmethod (shoot, spectral or both): dict(found, ntheta,

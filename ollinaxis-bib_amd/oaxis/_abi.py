@@ -128,6 +128,8 @@ def load_library(path=LIBPATH):
         "oa_checkpoint_size": (ctypes.c_int, [H, ctypes.POINTER(ctypes.c_size_t)]),
         "oa_checkpoint_save": (ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_size_t]),
         "oa_checkpoint_load": (ctypes.c_int, [H, ctypes.c_void_p, ctypes.c_size_t]),
+        "oa_checkpoint_restart": (ctypes.c_int, [H, ctypes.c_char_p]),
+        "oa_checkpoint_write": (ctypes.c_int, [H, ctypes.c_char_p, ctypes.c_char_p]),
         "oa_params_footprint": (ctypes.c_int, [P, dp]),
         "oa_batch_create": (ctypes.c_int, [ctypes.POINTER(H)]),
         "oa_batch_destroy": (ctypes.c_int, [H]),

@@ -130,6 +130,8 @@ def test_supported_configurations():
     assert supported("order = four\nshift = Gammadriver2\nmattertype = scalar\n" + size)[0] == 0
     assert supported("order = four\nidata = BrillWave\n" + size)[0] == 0
     assert supported("order = four\nmattertype = scalar\nidata = scalarpulse\n" + size)[0] == 0
+    # a restart from the reference's checkpoint directory (oa_checkpoint_restart)
+    assert supported("order = four\nidata = checkpoint\n" + size)[0] == 0
     # the reference's default 10 x 10 grid is below one 16 x 16 tile
     assert supported("order = four\n")[0] != 0
 
@@ -143,7 +145,6 @@ def test_supported_configurations():
     # initial data the library does not build, and solves it does not run
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = kerr\n", "idata"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = testgw\n", "idata"),
-    ("order = four\nNrtotal = 32\nNztotal = 32\nidata = checkpoint\n", "idata"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = complexpulse\n", "idata"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = bosonstar\n", "idata"),
     ("order = four\nNrtotal = 32\nNztotal = 32\nidata = rotboson\n", "idata"),
