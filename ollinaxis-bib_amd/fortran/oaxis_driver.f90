@@ -1,6 +1,9 @@
 ! Minimal Fortran main loop over liboaxis.so: what main.f90 + evolve.f90 of
 ! the reference do for the evolution, with the substep body replaced by the
-! C ABI.  Usage:  oaxis_driver <parfile> <coarse steps> <output file>
+! C ABI.  Usage:  oaxis_driver <parfile> <coarse steps> <output file> [<checkpoint directory>]
+! With a checkpoint directory (and idata = checkpoint in the par file) the
+! state comes from the reference's checkpoint files, as initial.f90:157-161
+! calls checkpointrestart, instead of the initial data.
 ! Writes alpha, chi and trK of level 0 as raw doubles in the reference
 ! layout (1-ghost:Nr, 1-ghost:Nz), column-major, prints a summary line, then
 ! looks for the apparent horizon at the final time (horizon_finder, as
@@ -11,7 +14,7 @@ program oaxis_driver
   implicit none
   type(oaxis_params), target :: par
   type(c_ptr) :: h
-  character(len=1024) :: parfile, arg, outfile
+  character(len=1024) :: parfile, arg, outfile, ckdir
   integer(c_int) :: nsteps, Nr, Nz, ghost, nlev
   real(c_double) :: dr, dz, dt, t, t1, t2, ms
   integer(c_int) :: s
@@ -24,6 +27,8 @@ program oaxis_driver
   call get_command_argument(2, arg)
   read (arg, *) nsteps
   call get_command_argument(3, outfile)
+  ckdir = ''
+  if (command_argument_count() >= 4) call get_command_argument(4, ckdir)
 
   call oaxis_params_init(par)
   call oaxis_check(oa_params_parse_file(c_loc(par%bytes), c_str(parfile)), "oa_params_parse_file")
@@ -34,7 +39,11 @@ program oaxis_driver
   call oaxis_check(oa_num_levels(h, nlev), "oa_num_levels")
   call oaxis_check(oa_level_info(h, 0_c_int, 0_c_int, Nr, Nz, ghost, dr, dz, dt), "oa_level_info")
 
-  call oaxis_check(oa_initial_data(h), "oa_initial_data")
+  if (len_trim(ckdir) > 0) then
+    call oaxis_check(oa_checkpoint_restart(h, c_str(ckdir)), "oa_checkpoint_restart")
+  else
+    call oaxis_check(oa_initial_data(h), "oa_initial_data")
+  end if
   call oaxis_check(oa_initial_aux(h), "oa_initial_aux")
   call oaxis_check(oa_evolve(h, nsteps), "oa_evolve")
   call oaxis_check(oa_last_elapsed_ms(h, ms), "oa_last_elapsed_ms")

@@ -76,6 +76,39 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path, oaxis_mod, oracle_mo
     orc.close()
 
 
+def test_fortran_driver_restarts_from_a_reference_checkpoint(tmp_path, oaxis_mod):
+    """The driver with idata = checkpoint reads the reference's checkpoint
+    directory (oa_checkpoint_restart) and continues exactly as the same
+    restart driven from Python."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (run __graft_entry__.build())")
+    p = oaxis_mod.params_from_text(PAR)
+    ck = tmp_path / "checkpoint_t=0.0750"
+    with oaxis_mod.Spacetime(p) as st:
+        st.initial_data()
+        st.initial_aux()
+        st.evolve(STEPS)
+        st.checkpoint_write(str(ck), "schwarz.par")
+    text = PAR.replace("idata = schwarzschild", "idata = checkpoint")
+    par = tmp_path / "restart.par"
+    par.write_text(text)
+    out = tmp_path / "out.bin"
+    print("fortran driver: restart from %s, %d more steps" % (ck.name, STEPS), flush=True)
+    res = subprocess.run([DRIVER, str(par), str(STEPS), str(out), str(ck)], capture_output=True, text=True,
+                         timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    with oaxis_mod.Spacetime(oaxis_mod.params_from_text(text)) as st:
+        st.restart(str(ck))
+        st.initial_aux()
+        st.evolve(STEPS)
+        shape = st.shape(0)
+        want = {n: st.download(n, 0) for n in ("alpha", "chi", "trK")}
+    got = np.fromfile(out, dtype=np.float64).reshape(3, shape[1], shape[0])
+    for k, n in enumerate(("alpha", "chi", "trK")):
+        assert np.array_equal(got[k].T, want[n]), n
+
+
 ONESTEP = os.path.join(ROOT, "ollinaxis-bib_amd", "fortran", "build", "oaxis_onestep_driver")
 
 PIECEWISE_PAR = """
