@@ -124,6 +124,7 @@ def cpu_baseline(texts, min_seconds, threads=1, states=None):
         owned = [[t % len(texts)] for t in range(threads)]
     orcs = [None] * len(owned)
     tally = [(0.0, 0)] * len(owned)
+    span = [None] * len(owned)          # each worker's own start and end of stepping
     start = threading.Barrier(len(owned) + 1)
 
     errors = []
@@ -148,6 +149,7 @@ def cpu_baseline(texts, min_seconds, threads=1, states=None):
             steps += 1
             k += 1
         tally[t] = (updates, steps)
+        span[t] = (t0, time.perf_counter())
 
     pool = [threading.Thread(target=worker, args=(t,)) for t in range(len(owned))]
     for th in pool:
@@ -156,15 +158,15 @@ def cpu_baseline(texts, min_seconds, threads=1, states=None):
         start.wait()
     except threading.BrokenBarrierError:
         pass
-    t0 = time.perf_counter()
     for th in pool:
         th.join()
-    dt = time.perf_counter() - t0
     for mine in orcs:
         for orc, _ in mine or []:
             orc.close()
     if errors:
         raise errors[0]
+    # from the first worker's start to the last one's end
+    dt = max(e for _, e in span) - min(b for b, _ in span)
     return sum(u for u, _ in tally) / dt, dt, sum(n for _, n in tally)
 
 
@@ -238,7 +240,8 @@ def run_serving(args, n, steps_scale, max_grid, plain=False):
 def horizon_probe(with_oracle):
     """One apparent-horizon search (shooting method) on Schwarzschild M=2,
     2 levels of 128x256: device time of oa_ahfind (second of two calls) and,
-    beside it, the oracle's search on the same state (one core)."""
+    beside it, the oracle's search on the same state (one core); then the
+    spectral method's first and second search on that state."""
     import oaxis
     from oaxis import workloads
     text = workloads.schwarzschild(2.0, 128, 256, 1)
@@ -264,6 +267,32 @@ def horizon_probe(with_oracle):
             out["oracle_matches"] = bool(ro["shots"] == r["shots"] and ro["zinit"] == r["zinit"]
                                          and ro["found"] == r["found"])
             orc.close()
+    # the spectral finder on the same state: the first search flows from a
+    # large circle, the next one starts from the horizon it found
+    ps = oaxis.params_from_text(text + "ahmethod = spectral\n")
+    sp = {"workload": "the same state, spectral method (ahNmodes 10, ahepsilon_spectral 1e-6)"}
+    with oaxis.Spacetime(ps) as st:
+        st.initial_data()
+        st.initial_aux()
+        t0 = time.perf_counter()
+        r = st.ahfind(0)
+        sp.update(device_ms=(time.perf_counter() - t0) * 1e3, found=r["found"], iterations=r["iterations"],
+                  mass=r["mass"])
+        t0 = time.perf_counter()
+        r2 = st.ahfind(0)
+        sp.update(again_ms=(time.perf_counter() - t0) * 1e3, again_iterations=r2["iterations"])
+        if with_oracle:
+            import pyoracle
+            orc = pyoracle.Oracle(ps)
+            orc.initial_data()
+            orc.initial_aux()
+            t0 = time.perf_counter()
+            ro = orc.ahfind(0)
+            sp["oracle_ms"] = (time.perf_counter() - t0) * 1e3
+            sp["oracle_matches"] = bool(ro["found"] == r["found"] and abs(ro["iterations"] - r["iterations"]) <= 1
+                                        and abs(ro["mass"] - r["mass"]) <= 1e-9 * max(abs(ro["mass"]), 1.0))
+            orc.close()
+    out["spectral"] = sp
     return out
 
 
