@@ -39,6 +39,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -193,6 +194,10 @@ struct AhCell {
   double fa[4][4];       // quantity f of the records (i0-1+a, j0-1+b), indices clamped
 };
 
+// BCAST: every lane of the half-wave receives the 20 values in v (through its
+// slot in sv, with two block barriers); otherwise the values are only left
+// in the half-wave's slot of sv, and no barrier is taken.
+template <bool BCAST = true>
 __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N], double* sv,
                           AhCell& cell) {
   const KGrid& G = L.G;
@@ -264,6 +269,7 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
     }
   }
   if (f < AF_N) slot[f] = out;
+  if (!BCAST) return;
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < AF_N; ++q) v[q] = slot[q];
@@ -272,16 +278,19 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
 
 // ------------------------------------------------------------ ah_source
 // h'' (sdh) and the area element (sarea) at (h, h', theta) for a horizon
-// centred at zcenter (:1983-2752).  Returns the error flag.
-__device__ int ah_source(const AhCtx& c, double rad, double dh, double th, double zcenter, double& sdh,
-                         double& sarea, double* sv, AhCell& cell) {
+// centred at zcenter (:1983-2752), in three parts: ah_locate (the checks
+// that end the reference's routine early, and the level to interpolate
+// on), ah_interp (the 20 quantities, one half-wave) and ah_source_math (the
+// rest, on one lane's values).
+struct AhPos {
+  double rc, zc, r0, z0;
+  int lev, err;
+};
+
+__device__ AhPos ah_locate(const AhCtx& c, double rad, double th, double zcenter) {
   const AhLevel& L0 = c.L[0];
-  const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
-  const double smallpi = M_PI;
-  sdh = 0.0;
-  // The checks that end the reference's ah_source early set err here.  Every
-  // lane still runs one interpolation (at a harmless point when err is set)
-  // so that both half-waves of a shot always meet its barriers together.
+  // the checks that end the reference's ah_source early set err; the
+  // interpolation point then moves to the middle of level 0
   int err = (rad < 0.0) ? 1 : 0;                                                 // :2140-2145
   const double rc = rad * std::sin(th);                                          // :2151-2159
   const double zc = rad * std::cos(th);
@@ -305,9 +314,20 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
     const AhLevel& L = c.L[ll];
     if (r0 > L.rmin + L.G.dr && r0 < L.rmax - L.G.dr && z0 > L.zmin + L.G.dz && z0 < L.zmax - L.G.dz) lev = ll;
   }
-  double v[AF_N];
-  ah_interp(c.L[lev], c.interporder, r0, z0, v, sv, cell);
-  if (err) return 1;
+  AhPos P;
+  P.rc = rc;
+  P.zc = zc;
+  P.r0 = r0;
+  P.z0 = z0;
+  P.lev = lev;
+  P.err = err;
+  return P;
+}
+
+__device__ int ah_source_math(double rad, double dh, double th, double rc, double zc, const double (&v)[AF_N],
+                              double& sdh, double& sarea) {
+  const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
+  const double smallpi = M_PI;
 
   const double interp_psi4 = (v[AF_psi] * v[AF_psi]) * (v[AF_psi] * v[AF_psi]);
   double Dphi[3];                                                                // :2247-2249
@@ -440,6 +460,19 @@ __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, doubl
   sdh = s;
   sarea = two * smallpi * interp_psi4 * std::sqrt(std::fabs(gd[2][2] * (dh * dh * gd[0][0] + two * dh * gd[0][1] + gd[1][1])));
   return 0;
+}
+
+// Returns the error flag.  Every lane runs one interpolation, at a harmless
+// point when the position checks fail, so that both half-waves of a shot
+// always meet its barriers together.
+__device__ int ah_source(const AhCtx& c, double rad, double dh, double th, double zcenter, double& sdh,
+                         double& sarea, double* sv, AhCell& cell) {
+  sdh = 0.0;
+  const AhPos P = ah_locate(c, rad, th, zcenter);
+  double v[AF_N];
+  ah_interp(c.L[P.lev], c.interporder, P.r0, P.z0, v, sv, cell);
+  if (P.err) return 1;
+  return ah_source_math(rad, dh, th, P.rc, P.zc, v, sdh, sarea);
 }
 
 // ------------------------------------------------------------ one shot
@@ -620,16 +653,20 @@ __global__ void __launch_bounds__(kShotBlock) k_ah_dzphi(const AhCtx* __restrict
 // ------------------------------------------------------------ spectral finder
 // ahspectral (:1113-1972): the fast flow of the cosine coefficients of the
 // surface r = h(theta) (Gundlach, gr-qc/9707050).  One workgroup runs one
-// search to the end: each stage evaluates ah_source at the Ntheta + 1 points
-// of the surface (one point per half-wave, kSpecBlock / 32 points per
-// round), one lane per mode sums the trapezoid coefficients in the
-// reference's order, every lane rebuilds a slice of h and h' from the
+// search to the end.  Each stage evaluates ah_source at the Ntheta + 1
+// points of the surface in two passes: the interpolation of the 20
+// quantities, one point per half-wave (kSpecBlock / 32 points per round,
+// the lanes sharing the stencil loads), into LDS; then the rest of the
+// source, a long scalar chain, one lane per point, so that it is not run
+// 32 times over.  One lane per mode then sums the trapezoid coefficients in
+// the reference's order, every lane rebuilds a slice of h and h' from the
 // coefficients, and lane 0 runs the residual and the step control.  The
 // surface, the coefficients and their previous values stay in LDS for the
 // whole search, and nothing returns to the host between iterations.
 constexpr int kSpecModes = 50;                      // ac(0:50) (:1165-1168)
 constexpr int kSpecThetaMax = 10 * kSpecModes;      // Ntheta = 10 Nmode (:1232)
-constexpr int kSpecBlock = 512;                     // 16 surface points per round
+constexpr int kSpecBlock = 512;                     // 16 surface points per interpolation round
+constexpr int kSpecChunk = 128;                     // surface points per source pass (<= kSpecBlock)
 
 struct SpecIn {
   const AhCtx* ctx;
@@ -648,9 +685,17 @@ struct SpecOut {
   int iter, err;
 };
 
+// The coefficient sums and the rebuilds read the cos / sin tables term after
+// term; TAB_LDS copies them into LDS first (when they fit, kSpecTabLds), so
+// each term waits for an LDS read instead of an L2 one.
+constexpr int kSpecTabLds = 64 * 1024;
+
+template <bool TAB_LDS>
 __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __restrict__ in,
                                                             SpecOut* __restrict__ out) {
-  __shared__ double sv[kSpecBlock / 32 * AF_N];
+  extern __shared__ double tab[];          // TAB_LDS: cos(j theta_i), then sin(j theta_i)
+  __shared__ double vals[(kSpecChunk + kSpecBlock / 32) * AF_N];   // interpolated quantities of a chunk
+  __shared__ int perr[kSpecChunk];
   __shared__ double rad[kSpecThetaMax + 1], drad[kSpecThetaMax + 1];
   __shared__ double rad_p[kSpecThetaMax + 1], drad_p[kSpecThetaMax + 1];
   __shared__ double Ifunc[kSpecThetaMax + 1], area[kSpecThetaMax + 1];
@@ -665,15 +710,24 @@ __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __rest
   const double zero = 0.0, half = 0.5, one = 1.0, two = 2.0;
   const double smallpi = M_PI;
   const double dtheta = s.dtheta, BB = s.BB, AA0 = s.AA, zcenter = s.zcenter;
-  const double* __restrict__ cs = s.cs;
-  const double* __restrict__ sn = s.sn;
+  const int ntab = (nm + 1) * (N + 1);
+  if (TAB_LDS)
+    for (int k = t; k < ntab; k += nt) {
+      tab[k] = s.cs[k];
+      tab[ntab + k] = s.sn[k];
+    }
+  const double* __restrict__ cs_g = s.cs;
+  const double* __restrict__ sn_g = s.sn;
+  auto CS = [&](int k) { return TAB_LDS ? tab[k] : cs_g[k]; };
+  auto SN = [&](int k) { return TAB_LDS ? tab[ntab + k] : sn_g[k]; };
   // h(theta_i), h'(theta_i) from the coefficients (:1320-1327, :1602-1609)
   auto rebuild = [&]() {
     for (int i = t; i <= N; i += nt) {
       double r = half * ac[0], d = zero;
+#pragma unroll 4
       for (int j = step; j <= nm; j += step) {
-        r = r + ac[j] * cs[j * (N + 1) + i];
-        d = d - static_cast<double>(j) * ac[j] * sn[j * (N + 1) + i];
+        r = r + ac[j] * CS(j * (N + 1) + i);
+        d = d - static_cast<double>(j) * ac[j] * SN(j * (N + 1) + i);
       }
       rad[i] = r;
       drad[i] = d;
@@ -714,20 +768,38 @@ __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __rest
     if (t == 0) x_err = 0;
     __syncthreads();
     for (int q = 1; q <= 2; ++q) {
-      // sources at every point (:1494-1501); the rounds' spare half-waves
-      // evaluate the last point again and keep nothing
-      for (int base = 0; base <= N; base += nhw) {
-        const int i = base + hw;
-        const int ii = i <= N ? i : N;
-        double sdh, sarea;
-        const int e = ah_source(c, rad[ii], drad[ii], static_cast<double>(ii) * dtheta, zcenter, sdh, sarea, sv, cell);
-        if ((t & 31) == 0 && i <= N) {
+      // sources at every point (:1494-1501), kSpecChunk points at a time:
+      // the 20 quantities of each point by one half-wave (the rounds' spare
+      // half-waves interpolate the chunk's last point again into spare
+      // slots), then the rest of ah_source by one lane per point
+      for (int c0 = 0; c0 <= N; c0 += kSpecChunk) {
+        const int cn = (N + 1 - c0 < kSpecChunk) ? N + 1 - c0 : kSpecChunk;
+        for (int b = 0; b < cn; b += nhw) {
+          const int k = b + hw;
+          const int i = c0 + (k < cn ? k : cn - 1);
+          const AhPos P = ah_locate(c, rad[i], static_cast<double>(i) * dtheta, zcenter);
+          double vnone[AF_N];
+          ah_interp<false>(c.L[P.lev], c.interporder, P.r0, P.z0, vnone, vals + b * AF_N, cell);
+          if ((t & 31) == 0 && k < cn) perr[k] = P.err;
+        }
+        __syncthreads();
+        if (t < cn) {
+          const int i = c0 + t;
+          const double th = static_cast<double>(i) * dtheta;
+          double sdh = 0.0, sarea = 0.0;
+          int e = perr[t];
+          if (!e) {
+            double v[AF_N];
+#pragma unroll
+            for (int f = 0; f < AF_N; ++f) v[f] = vals[t * AF_N + f];
+            e = ah_source_math(rad[i], drad[i], th, rad[i] * std::sin(th), rad[i] * std::cos(th), v, sdh, sarea);
+          }
           Ifunc[i] = sdh;
           area[i] = sarea;
           if (e) x_err = 1;
         }
+        __syncthreads();
       }
-      __syncthreads();
       err = x_err;
       if (err) break;                                                            // goto 100
       if (t == 0) {                                                              // :1506-1515
@@ -740,10 +812,12 @@ __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __rest
       }
       __syncthreads();
       if (jm <= nm) {                                                            // :1539-1563, :1590-1592
-        const double* cj = cs + jm * (N + 1);
+        const int cj = jm * (N + 1);
         double cc_j = half * dtheta * Ifunc[0];
-        for (int i = 1; i < N; ++i) cc_j = cc_j + dtheta * Ifunc[i] * cj[i];
-        cc_j = cc_j + half * dtheta * Ifunc[N] * cj[N];
+        // unrolled so that the reads run ahead of the (serial) sum
+#pragma unroll 8
+        for (int i = 1; i < N; ++i) cc_j = cc_j + dtheta * Ifunc[i] * CS(cj + i);
+        cc_j = cc_j + half * dtheta * Ifunc[N] * CS(cj + N);
         cc_j = two * static_cast<double>(step) * cc_j / smallpi;
         const double jj = static_cast<double>(jm) * static_cast<double>(jm);
         cc_j = cc_j + jj * ac[jm];
@@ -1478,8 +1552,20 @@ int run_flows(std::vector<Job*>& js, hipStream_t st) {
     }
   }
   const int n = static_cast<int>(js.size());
+  size_t tab_bytes = 0;
+  for (int q = 0; q < n; ++q) {
+    const SpecIn& in = buf.h_sin[q];
+    tab_bytes = std::max(tab_bytes, 2 * sizeof(double) * static_cast<size_t>(in.nmode + 1) * (in.ntheta + 1));
+  }
   AH_HIP(hipMemcpyAsync(buf.d_sin, buf.h_sin, sizeof(SpecIn) * n, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_ah_spectral, dim3(n), dim3(kSpecBlock), 0, st, buf.d_sin, buf.d_sout);
+  if (tab_bytes <= static_cast<size_t>(kSpecTabLds)) {
+    // the tables and the kernel's own LDS together pass the 64 KiB default
+    AH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_ah_spectral<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, kSpecTabLds));
+    hipLaunchKernelGGL(k_ah_spectral<true>, dim3(n), dim3(kSpecBlock), tab_bytes, st, buf.d_sin, buf.d_sout);
+  } else {
+    hipLaunchKernelGGL(k_ah_spectral<false>, dim3(n), dim3(kSpecBlock), 0, st, buf.d_sin, buf.d_sout);
+  }
   AH_HIP(hipGetLastError());
   AH_HIP(hipMemcpyAsync(buf.h_sout, buf.d_sout, sizeof(SpecOut) * n, hipMemcpyDeviceToHost, st));
   AH_HIP(hipStreamSynchronize(st));
