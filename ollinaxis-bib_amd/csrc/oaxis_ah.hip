@@ -194,12 +194,18 @@ struct AhCell {
   double fa[4][4];       // quantity f of the records (i0-1+a, j0-1+b), indices clamped
 };
 
-// BCAST: every lane of the half-wave receives the 20 values in v (through its
-// slot in sv, with two block barriers); otherwise the values are only left
-// in the half-wave's slot of sv, and no barrier is taken.
-template <bool BCAST = true>
-__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N], double* sv,
-                          AhCell& cell) {
+// How interp.f90 combines the 4 x 4 stencil records around (r0, z0) on one
+// level (:146-263): the cell, and a copy at a node, the bilinear formula
+// near the level's edges (or with interporder = bilinear), else bicubic
+// Lagrange weights.  Formed once per point; ah_combine applies it to the
+// records of one quantity with the reference's expressions.
+struct AhPlan {
+  int lev, i0, j0, mode;              // mode 0: node copy, 1: bilinear, 2: bicubic
+  double auxr, auxz;
+  double w[4][4];
+};
+
+__device__ __forceinline__ void ah_plan(const AhLevel& L, int order, double r0, double z0, AhPlan& P) {
   const KGrid& G = L.G;
   const int g = G.g;
   const double r1g = (static_cast<double>(1 - g) - 0.5) * G.dr;
@@ -208,68 +214,88 @@ __device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, dou
   const double deltar = r0 - (static_cast<double>(i0) - 0.5) * G.dr;
   const int j0 = static_cast<int>((z0 - z1g) / G.dz) + 1 - g;
   const double deltaz = z0 - (static_cast<double>(j0) - G.zoff) * G.dz;
+  P.i0 = i0;
+  P.j0 = j0;
+  P.auxr = 0.0;
+  P.auxz = 0.0;
+  if (deltar == 0.0 && deltaz == 0.0) {
+    P.mode = 0;
+    return;
+  }
+  const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
+                   ((z0 - L.zmin) < G.dz);
+  const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
+  P.auxr = auxr;
+  P.auxz = auxz;
+  if (order == OA_INTERP_BILINEAR || lin) {
+    P.mode = 1;
+    return;
+  }
+  P.mode = 2;
+#pragma unroll
+  for (int i = -1; i <= 2; ++i) {
+    double wr = 1.0;
+#pragma unroll
+    for (int m = -1; m <= 2; ++m) {
+      if (m == i) continue;
+      wr = wr * (auxr - static_cast<double>(m)) / static_cast<double>(i - m);
+    }
+#pragma unroll
+    for (int j = -1; j <= 2; ++j) {
+      double aux = wr;
+#pragma unroll
+      for (int n = -1; n <= 2; ++n) {
+        if (n == j) continue;
+        aux = aux * (auxz - static_cast<double>(n)) / static_cast<double>(j - n);
+      }
+      P.w[i + 1][j + 1] = aux;
+    }
+  }
+}
+
+__device__ __forceinline__ double ah_combine(const AhPlan& P, const double (&fa)[4][4]) {
+  if (P.mode == 0) return fa[1][1];
+  const double auxr = P.auxr, auxz = P.auxz;
+  if (P.mode == 1)
+    return fa[1][1] * (1.0 - auxr) * (1.0 - auxz) + fa[2][1] * auxr * (1.0 - auxz) + fa[1][2] * (1.0 - auxr) * auxz +
+           fa[2][2] * auxr * auxz;
+  double out = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out = out + P.w[i][j] * fa[i][j];
+  return out;
+}
+
+// quantity f of the 16 records of the plan's cell, indices clamped to the
+// level's arrays
+__device__ __forceinline__ void ah_records(const AhLevel& L, int i0, int j0, int f, double (&fa)[4][4]) {
+  const KGrid& G = L.G;
+  const int g = G.g;
   auto ci = [&](int i) { return i < 1 - g ? 1 - g : (i > G.Nr ? G.Nr : i); };
   auto cj = [&](int j) { return j < 1 - g ? 1 - g : (j > G.Nz ? G.Nz : j); };
-  // each half-wave evaluates its own point (see ah_integrate); lane f of a
-  // half-wave sums quantity f
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) fa[i][j] = L.q[IDX(G, ci(i0 - 1 + i), cj(j0 - 1 + j)) * AF_N + f];
+}
+
+// The 20 quantities at (r0, z0) on level L by one half-wave (lane f sums
+// quantity f); every lane of the half-wave receives them in v, through its
+// slot in sv and two block barriers.
+__device__ void ah_interp(const AhLevel& L, int order, double r0, double z0, double (&v)[AF_N], double* sv,
+                          AhCell& cell) {
+  AhPlan P;
+  ah_plan(L, order, r0, z0, P);
   const int f = static_cast<int>(threadIdx.x & 31u);
   double* slot = sv + (threadIdx.x >> 5) * AF_N;
-  // quantity f of one point record
-  auto at = [&](int i, int j) { return L.q[IDX(G, ci(i), cj(j)) * AF_N + f]; };
-  if (cell.q != L.q || cell.i0 != i0 || cell.j0 != j0) {
-    if (f < AF_N) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cell.fa[i][j] = at(i0 - 1 + i, j0 - 1 + j);
-    }
+  if (cell.q != L.q || cell.i0 != P.i0 || cell.j0 != P.j0) {
+    if (f < AF_N) ah_records(L, P.i0, P.j0, f, cell.fa);
     cell.q = L.q;
-    cell.i0 = i0;
-    cell.j0 = j0;
+    cell.i0 = P.i0;
+    cell.j0 = P.j0;
   }
-  const auto& fa = cell.fa;
-  double out = 0.0;
-  if (deltar == 0.0 && deltaz == 0.0) {
-    if (f < AF_N) out = fa[1][1];
-  } else {
-    const bool lin = ((L.rmax - r0) < G.dr) || ((L.zmax - z0) < G.dz) || ((r0 - L.rmin) < G.dr) ||
-                     ((z0 - L.zmin) < G.dz);
-    const double auxr = deltar / G.dr, auxz = deltaz / G.dz;
-    if (order == OA_INTERP_BILINEAR || lin) {
-      if (f < AF_N)
-        out = fa[1][1] * (1.0 - auxr) * (1.0 - auxz) + fa[2][1] * auxr * (1.0 - auxz) +
-              fa[1][2] * (1.0 - auxr) * auxz + fa[2][2] * auxr * auxz;
-    } else {
-      double w[4][4];
-#pragma unroll
-      for (int i = -1; i <= 2; ++i) {
-        double wr = 1.0;
-#pragma unroll
-        for (int m = -1; m <= 2; ++m) {
-          if (m == i) continue;
-          wr = wr * (auxr - static_cast<double>(m)) / static_cast<double>(i - m);
-        }
-#pragma unroll
-        for (int j = -1; j <= 2; ++j) {
-          double aux = wr;
-#pragma unroll
-          for (int n = -1; n <= 2; ++n) {
-            if (n == j) continue;
-            aux = aux * (auxz - static_cast<double>(n)) / static_cast<double>(j - n);
-          }
-          w[i + 1][j + 1] = aux;
-        }
-      }
-      if (f < AF_N) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) out = out + w[i][j] * fa[i][j];
-      }
-    }
-  }
-  if (f < AF_N) slot[f] = out;
-  if (!BCAST) return;
+  if (f < AF_N) slot[f] = ah_combine(P, cell.fa);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < AF_N; ++q) v[q] = slot[q];
@@ -694,7 +720,8 @@ template <bool TAB_LDS>
 __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __restrict__ in,
                                                             SpecOut* __restrict__ out) {
   extern __shared__ double tab[];          // TAB_LDS: cos(j theta_i), then sin(j theta_i)
-  __shared__ double vals[(kSpecChunk + kSpecBlock / 32) * AF_N];   // interpolated quantities of a chunk
+  __shared__ double vals[kSpecChunk * AF_N];          // interpolated quantities of a chunk
+  __shared__ AhPlan plans[kSpecChunk];                // their interpolation plans
   __shared__ int perr[kSpecChunk];
   __shared__ double rad[kSpecThetaMax + 1], drad[kSpecThetaMax + 1];
   __shared__ double rad_p[kSpecThetaMax + 1], drad_p[kSpecThetaMax + 1];
@@ -747,9 +774,6 @@ __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __rest
     }
   }
   for (int i = t; i <= N; i += nt) Ifunc[i] = area[i] = zero;
-  AhCell cell;
-  cell.q = nullptr;
-  cell.i0 = cell.j0 = 0;
   const int hw = t >> 5, nhw = nt >> 5;
   const int jm = t * step;                 // this lane's mode in the coefficient sums
   double res = 1.0, AA = AA0;
@@ -768,19 +792,32 @@ __global__ void __launch_bounds__(kSpecBlock) k_ah_spectral(const SpecIn* __rest
     if (t == 0) x_err = 0;
     __syncthreads();
     for (int q = 1; q <= 2; ++q) {
-      // sources at every point (:1494-1501), kSpecChunk points at a time:
-      // the 20 quantities of each point by one half-wave (the rounds' spare
-      // half-waves interpolate the chunk's last point again into spare
-      // slots), then the rest of ah_source by one lane per point
+      // sources at every point (:1494-1501), kSpecChunk points at a time, in
+      // three passes: each point's position checks, level and interpolation
+      // plan (one lane per point); its 20 quantities (one half-wave per
+      // point, the lanes sharing the stencil loads); the rest of ah_source
+      // (one lane per point)
       for (int c0 = 0; c0 <= N; c0 += kSpecChunk) {
         const int cn = (N + 1 - c0 < kSpecChunk) ? N + 1 - c0 : kSpecChunk;
+        if (t < cn) {
+          const int i = c0 + t;
+          const AhPos P = ah_locate(c, rad[i], static_cast<double>(i) * dtheta, zcenter);
+          AhPlan pl;
+          ah_plan(c.L[P.lev], c.interporder, P.r0, P.z0, pl);
+          pl.lev = P.lev;
+          plans[t] = pl;
+          perr[t] = P.err;
+        }
+        __syncthreads();
         for (int b = 0; b < cn; b += nhw) {
           const int k = b + hw;
-          const int i = c0 + (k < cn ? k : cn - 1);
-          const AhPos P = ah_locate(c, rad[i], static_cast<double>(i) * dtheta, zcenter);
-          double vnone[AF_N];
-          ah_interp<false>(c.L[P.lev], c.interporder, P.r0, P.z0, vnone, vals + b * AF_N, cell);
-          if ((t & 31) == 0 && k < cn) perr[k] = P.err;
+          const int f = t & 31;
+          if (k < cn && f < AF_N) {
+            const AhPlan& pl = plans[k];
+            double fa[4][4];
+            ah_records(c.L[pl.lev], pl.i0, pl.j0, f, fa);
+            vals[k * AF_N + f] = ah_combine(pl, fa);
+          }
         }
         __syncthreads();
         if (t < cn) {
