@@ -323,8 +323,9 @@ int oa_checkpoint_load(oa_handle* h, const void* buf, size_t bytes);
  * reference also lists but this path does not form (dtalpha, KTH, lambda,
  * hdet, ...) are skipped.  Then call oa_initial_aux, as evolve.f90:128-195
  * runs symmetries, the auxiliary pass and the Gamma-driver initialisation
- * after initial.  oa_initial_data refuses idata = checkpoint
- * (OA_ERR_STATE) and points here. */
+ * after initial.  The horizon finder starts afresh, as in a new run of the
+ * reference.  oa_initial_data refuses idata = checkpoint (OA_ERR_STATE) and
+ * points here. */
 int oa_checkpoint_restart(oa_handle* h, const char* dir);
 /* checkpointsave (checkpoint.f90:2-328) for one process into dir (created
  * if missing; the reference names it <directory>/checkpoint_t=<t>):

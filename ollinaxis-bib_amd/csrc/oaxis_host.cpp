@@ -1131,8 +1131,15 @@ extern "C" int oa_checkpoint_write(oa_handle* h, const char* dir, const char* pa
 extern "C" int oa_checkpoint_restart(oa_handle* h, const char* dir) {
   if (!h || !dir) return OA_ERR_ARG;
   RC(ckpt_check(h, "oa_checkpoint_restart"));
+  if (ah_pending(h)) {
+    oa_set_error("oa_checkpoint_restart: a begun horizon search of this spacetime has not ended");
+    return OA_ERR_STATE;
+  }
   const std::string d(dir);
   RC(ckref_params(h, d));
+  // a restarted run is a new run of the reference: its horizon finder starts
+  // afresh (ahfound, ack and the first-call flags are not checkpointed)
+  ah_release(h);
   std::vector<std::string> names;                                  // checklist (:553-565)
   {
     TextFile in(d + "/checklist");
