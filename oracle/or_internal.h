@@ -5,8 +5,8 @@
  * the reference's whole-array style (one loop over the level per reference
  * statement) so that every step can be compared with the Fortran source it
  * follows (file:line in each function).  It is the checker for the HIP
- * library: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg may load it.  The product never links, loads or calls it.
+ * library: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * legs (the throughput sample and the horizon probe's search) may load it.  The product never links, loads or calls it.
  *
  * Arrays use the reference layout: Fortran column-major, rho index fastest,
  * index range (1-ghost:Nr, 1-ghost:Nz).

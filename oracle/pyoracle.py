@@ -2,7 +2,8 @@
 
 ctypes wrapper of liboaxis_oracle.so, the plain-C restatement of the
 reference's evolution hot path (see or_internal.h).  Loaded only by tests/,
-__graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+__graft_entry__.smoke() and bench.py's CPU-baseline legs (the throughput sample
+and the horizon probe's search), as the checker.
 """
 import ctypes
 import os
