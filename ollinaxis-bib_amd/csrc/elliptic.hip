@@ -188,12 +188,8 @@ struct EllIterArgs {
 // Kreiss-Oliger terms on v, the radiative boundary of v, the RK4
 // accumulators and the update, at every computed point with its parity
 // images (both fields are even on the axis and the equator).
-__global__ void __launch_bounds__(kBlk) k_ell_iter(EllIterArgs a, const EllCtl* __restrict__ ctl) {
-  if (ctl->done) return;
+__device__ __forceinline__ void ell_iter_point(const EllIterArgs& a, int i, int j) {
   const KGrid& G = a.L.G;
-  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-  if (j > G.Nz || i > G.Nr) return;
   const EllPlanes& E = a.L.P;
   const long k = IDX(G, i, j);
   const Plane U{a.uin, G}, V{a.vin, G};
@@ -237,6 +233,16 @@ __global__ void __launch_bounds__(kBlk) k_ell_iter(EllIterArgs a, const EllCtl* 
   if (a.base && a.iter == a.niter) E.sv[k] = sv;
 }
 
+// the base level's iteration (no fine boundary)
+__global__ void __launch_bounds__(kBlk) k_ell_iter(EllIterArgs a, const EllCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  const KGrid& G = a.L.G;
+  const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+  if (j > G.Nz || i > G.Nr) return;
+  ell_iter_point(a, i, j);
+}
+
 struct EllFbArgs {
   KGrid Gf, Gc;
   EllPlanes F;                       // fine level
@@ -250,11 +256,9 @@ struct EllFbArgs {
 // finebound(box, level, dtw, .false.) for u, then v (finebound.f90:70-400,
 // single-variable branches): the depth and the order of the time
 // interpolation come from the device clocks as the reference forms them.
-__global__ void __launch_bounds__(kBlk) k_ell_finebound(EllFbArgs a, const EllCtl* __restrict__ ctl) {
-  if (ctl->done) return;
+__device__ __forceinline__ void ell_fb_point(const EllFbArgs& a, long q) {
   const double tp = a.cclk->t, t0 = a.fclk->t, tm1 = a.fclk->t1, tl = t0 + a.dtw;
   const int tab = (tp == tl) ? 0 : 1;
-  const long q = static_cast<long>(blockIdx.x) * kBlk + threadIdx.x;
   if (q >= a.n[tab]) return;
   const FbPoint fp = a.tab[tab][q];
   if (fp.which < 0) return;
@@ -280,6 +284,37 @@ __global__ void __launch_bounds__(kBlk) k_ell_finebound(EllFbArgs a, const EllCt
             (tl - tp) * (tl - t0) / ((tm1 - tp) * (tm1 - t0)) * b1;
     }
     put(f == 0 ? a.uout : a.vout, Gf, i, j, val, 1.0, 1.0);
+  }
+}
+
+// One iteration of a fine level and its finebound in one launch.  The
+// finebound overwrites the zone of depth D (g+2 when the clocks meet, else
+// g-1: the rows i >= Nr-D, j >= Nz-D and, without equatorial symmetry,
+// j <= 1-g+D) of the iteration's output, point and parity images alike, and
+// reads nothing the iteration writes; so the iteration's blocks skip the
+// zone and the finebound's blocks fill it, with the same final values as
+// the two launches in sequence.
+struct EllStepArgs {
+  EllIterArgs it;
+  EllFbArgs fb;
+  int nbx, nb_iter;                  // iteration blocks: nbx along z (64 points), rows of 4 along rho
+};
+
+__global__ void __launch_bounds__(kBlk) k_ell_iter_fb(EllStepArgs a, const EllCtl* __restrict__ ctl) {
+  if (ctl->done) return;
+  const int b = static_cast<int>(blockIdx.x);
+  if (b < a.nb_iter) {
+    const KGrid& G = a.it.L.G;
+    const int j = G.jlo + (b % a.nbx) * 64 + static_cast<int>(threadIdx.x & 63u);
+    const int i = 1 + (b / a.nbx) * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (j > G.Nz || i > G.Nr) return;
+    const int g = G.g;
+    const double tp = a.fb.cclk->t, tl = a.fb.fclk->t + a.fb.dtw;
+    const int D = (tp == tl) ? g + 2 : g - 1;
+    if (i >= G.Nr - D || j >= G.Nz - D || (!G.eqsym && j <= 1 - g + D)) return;
+    ell_iter_point(a.it, i, j);
+  } else {
+    ell_fb_point(a.fb, static_cast<long>(b - a.nb_iter) * kBlk + threadIdx.x);
   }
 }
 
@@ -779,9 +814,10 @@ struct EllSolver {
       it.niter = niter;
       it.rk4 = rk4 ? 1 : 0;
       it.base = (level == 0) ? 1 : 0;
-      hipLaunchKernelGGL(k_ell_iter, grd, blk, 0, st, it, ctl);
-      ELLRC(launched("iteration"));
-      if (level > 0) {
+      if (level == 0) {
+        hipLaunchKernelGGL(k_ell_iter, grd, blk, 0, st, it, ctl);
+        ELLRC(launched("iteration"));
+      } else {
         const EllLevelArgs& c = lev[level - 1];
         const Level& L = h->L[level];
         EllFbArgs fb{};
@@ -799,9 +835,14 @@ struct EllSolver {
         fb.cclk = c.clk;
         fb.fclk = a.clk;
         fb.dtw = dtw;
-        hipLaunchKernelGGL(k_ell_finebound, dim3(blocks(std::max(L.fbn[0], L.fbn[1]))), dim3(kBlk), 0, st, fb,
-                           ctl);
-        ELLRC(launched("finebound"));
+        EllStepArgs sa{};
+        sa.it = it;
+        sa.fb = fb;
+        sa.nbx = static_cast<int>(grd.x);
+        sa.nb_iter = static_cast<int>(grd.x * grd.y);
+        const int nb = sa.nb_iter + blocks(std::max(L.fbn[0], L.fbn[1]));
+        hipLaunchKernelGGL(k_ell_iter_fb, dim3(nb), dim3(kBlk), 0, st, sa, ctl);
+        ELLRC(launched("iteration and finebound"));
       }
     }
     hipLaunchKernelGGL(k_ell_tick, dim3(1), dim3(64), 0, st, a.clk, G.dt, ctl);

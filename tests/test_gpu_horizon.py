@@ -201,16 +201,18 @@ Nztotal = 64
 """
 
 
-def test_serving_retires_on_horizon(oaxis_mod):
+@pytest.mark.parametrize("method", ["shoot", "spectral"])
+def test_serving_retires_on_horizon(oaxis_mod, method):
     """With ahafter > 0 the search at t = 0 is skipped, so the puncture's
     horizon counts as formed at the first search after ahafter: the search
     runs beside the next coarse step and the request retires after it with
     reason "horizon"; with ahafter = 0 the horizon exists from t = 0 and the
-    request runs to its last step."""
+    request runs to its last step.  Either finding method."""
     oaxis = oaxis_mod
     from oaxis.serve import Request, Server
-    late = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\nahafter = 0.1\n")
-    early = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\n")
+    m = "ahmethod = %s\n" % method
+    late = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\nahafter = 0.1\n" + m)
+    early = _schwarzschild_text(2.0, "ahfind = .true.\nahfind_every = 2\n" + m)
     reqs = [Request(0, late, 12, 0.0, "late"), Request(1, early, 6, 0.0, "early")]
     dt0 = oaxis.params_from_text(late).dtfac * 0.1
     records, summary = Server(1e12, check_every=0).run(reqs)
