@@ -51,12 +51,20 @@ struct EllClock {
 };
 
 // Outcome of the solve so far, evaluated after every coarse step.
+constexpr int kEllMaxLevels = 16;
 struct EllCtl {
   int step;        // coarse steps of the current phase
   int done;        // the stopping rule fired: later kernels return at once
   int stationary;  // 1: the all-levels phase may end on the stationary rule
   double gres;     // residual of the last step
   double wmax, wprev;   // stationary rule: window maxima of the relative change of u
+  // a level's clock advances at the end of its step (:997-1004).  The last
+  // iteration of the step marks it here, and the next saveold launched (of
+  // any level) applies it: every kernel that reads a level's clock (the
+  // iterations and finebounds of that level and of the next finer one)
+  // comes after a saveold, and no other kernel reads the clocks.
+  int tick[kEllMaxLevels];
+  double tick_dt[kEllMaxLevels];
 };
 
 // One level's planes for the solve.  u, v: the state at the end of a step
@@ -133,15 +141,27 @@ __global__ void __launch_bounds__(kBlk) k_ell_init(KGrid G, EllPlanes E, EllCloc
 }
 
 __global__ void k_ell_reset(EllCtl* ctl, int stationary, EllClock* clocks, int nlev) {
-  if (threadIdx.x == 0) *ctl = EllCtl{0, 0, stationary, 1.0, 0.0, 0.0};
+  if (threadIdx.x == 0) *ctl = EllCtl{0, 0, stationary, 1.0, 0.0, 0.0, {}, {}};
   if (static_cast<int>(threadIdx.x) < nlev) clocks[threadIdx.x] = EllClock{0.0, 0.0, 0.0, 0};
 }
 
 // ------------------------------------------------------------ one step
 // saveold of wavestep (:631-679): up = u, vp = v over the whole level, and
 // the strips shift one time level back (t-2 <- t-1 <- t0 <- now)
-__global__ void __launch_bounds__(kBlk) k_ell_saveold(EllLevelArgs a, int fine, const EllCtl* __restrict__ ctl) {
+__global__ void __launch_bounds__(kBlk) k_ell_saveold(EllLevelArgs a, int fine, EllCtl* ctl, EllClock* clocks,
+                                                     int nlev) {
   if (ctl->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int l = 0; l < nlev; ++l)
+      if (ctl->tick[l]) {                                     // the level clock's step (:997-1004)
+        EllClock c = clocks[l];
+        c.t2 = c.t1;
+        c.t1 = c.t;
+        c.s = c.s + 1;
+        c.t = c.t + ctl->tick_dt[l];
+        clocks[l] = c;
+        ctl->tick[l] = 0;
+      }
   const KGrid& G = a.G;
   const EllPlanes& E = a.P;
   const long q = static_cast<long>(blockIdx.x) * kBlk + threadIdx.x;
@@ -182,7 +202,17 @@ struct EllIterArgs {
   double* uout; double* vout;
   double dtw, w, idt, eta, diss;
   int iter, niter, rk4, base;
+  int level;
+  int s5;                            // S5 is not identically zero (a scalar potential)
 };
+
+// the last iteration of a level's step marks its clock for advancing
+__device__ __forceinline__ void ell_mark_tick(const EllIterArgs& a, EllCtl* ctl) {
+  if (a.iter == a.niter && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+    ctl->tick_dt[a.level] = a.L.G.dt;
+    ctl->tick[a.level] = 1;
+  }
+}
 
 // One internal iteration of one level (:691-915): sources of u and v, the
 // Kreiss-Oliger terms on v, the radiative boundary of v, the RK4
@@ -199,7 +229,11 @@ __device__ __forceinline__ void ell_iter_point(const EllIterArgs& a, int i, int 
   // flat Laplacian, the coefficients and the damping (:791-851)
   const double Dr_u = s_d1r(U, G, i, j), Drr_u = s_d2r(U, G, i, j), Dzz_u = s_d2z(U, G, i, j);
   double sv = Drr_u + Dzz_u + Dr_u / r;
-  sv = sv + E.s1[k] * u + E.s5[k] * pow5(u) - E.s0[k];
+  // S0 is zero for every supported initial data (subtracting +0 changes
+  // nothing) and S5 without a scalar potential (adding a zero changes the
+  // value of nothing either): their planes are read only when they matter
+  sv = sv + E.s1[k] * u;
+  if (a.s5) sv = sv + E.s5[k] * pow5(u);
   sv = sv - a.eta * v;
   // dissipation on v (dissipation.f90:181-199): rho terms for i <= Nr-3,
   // z terms for 1 <= j <= Nz-3
@@ -234,8 +268,9 @@ __device__ __forceinline__ void ell_iter_point(const EllIterArgs& a, int i, int 
 }
 
 // the base level's iteration (no fine boundary)
-__global__ void __launch_bounds__(kBlk) k_ell_iter(EllIterArgs a, const EllCtl* __restrict__ ctl) {
+__global__ void __launch_bounds__(kBlk) k_ell_iter(EllIterArgs a, EllCtl* ctl) {
   if (ctl->done) return;
+  ell_mark_tick(a, ctl);
   const KGrid& G = a.L.G;
   const int j = G.jlo + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   const int i = 1 + static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
@@ -300,8 +335,9 @@ struct EllStepArgs {
   int nbx, nb_iter;                  // iteration blocks: nbx along z (64 points), rows of 4 along rho
 };
 
-__global__ void __launch_bounds__(kBlk) k_ell_iter_fb(EllStepArgs a, const EllCtl* __restrict__ ctl) {
+__global__ void __launch_bounds__(kBlk) k_ell_iter_fb(EllStepArgs a, EllCtl* ctl) {
   if (ctl->done) return;
+  ell_mark_tick(a.it, ctl);
   const int b = static_cast<int>(blockIdx.x);
   if (b < a.nb_iter) {
     const KGrid& G = a.it.L.G;
@@ -318,16 +354,6 @@ __global__ void __launch_bounds__(kBlk) k_ell_iter_fb(EllStepArgs a, const EllCt
   }
 }
 
-// the level clock at the end of a level step (:997-1004)
-__global__ void k_ell_tick(EllClock* clk, double dt, const EllCtl* __restrict__ ctl) {
-  if (ctl->done || threadIdx.x != 0) return;
-  EllClock c = *clk;
-  c.t2 = c.t1;
-  c.t1 = c.t;
-  c.s = c.s + 1;
-  c.t = c.t + dt;
-  *clk = c;
-}
 
 struct EllRsArgs {
   KGrid Gf, Gc;
@@ -776,7 +802,8 @@ struct EllSolver {
     {
       const long per_f = static_cast<long>(G.g) * (G.Nz + G.g) + 2L * G.g * (G.Nr + G.g);
       const long n = std::max(G.plane, level > 0 ? 2 * per_f : 0L);
-      hipLaunchKernelGGL(k_ell_saveold, dim3(blocks(n)), dim3(kBlk), 0, st, a, level > 0 ? 1 : 0, ctl);
+      hipLaunchKernelGGL(k_ell_saveold, dim3(blocks(n)), dim3(kBlk), 0, st, a, level > 0 ? 1 : 0, ctl, clocks,
+                         nlev);
       ELLRC(launched("saveold"));
     }
     const dim3 blk(64, 4, 1);
@@ -814,6 +841,8 @@ struct EllSolver {
       it.niter = niter;
       it.rk4 = rk4 ? 1 : 0;
       it.base = (level == 0) ? 1 : 0;
+      it.level = level;
+      it.s5 = (p.idata == OA_IDATA_SCALARPULSE && p.scalarpotential != OA_POT_NONE) ? 1 : 0;
       if (level == 0) {
         hipLaunchKernelGGL(k_ell_iter, grd, blk, 0, st, it, ctl);
         ELLRC(launched("iteration"));
@@ -845,8 +874,6 @@ struct EllSolver {
         ELLRC(launched("iteration and finebound"));
       }
     }
-    hipLaunchKernelGGL(k_ell_tick, dim3(1), dim3(64), 0, st, a.clk, G.dt, ctl);
-    ELLRC(launched("clock"));
     if (level < nlmax) {
       ELLRC(wavestep(level + 1, nlmax));
       ELLRC(wavestep(level + 1, nlmax));
