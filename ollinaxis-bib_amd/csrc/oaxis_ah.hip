@@ -449,34 +449,44 @@ __device__ int ah_source_math(double rad, double dh, double th, double rc, doubl
   const double u = std::sqrt(std::fabs(u2));
   const double denom = (gu[0][0] * gu[1][1] - gu[0][1] * gu[0][1]) / (interp_psi4 * interp_psi4);
 
-  double Gam[2][3][3];                                                           // :2575-2608
+  // Christoffel symbols (:2575-2608) and the contractions (:2615-2650).  In
+  // axisymmetry without rotation g^{rho phi} = g^{z phi} = 0 and D_phi F = 0,
+  // so in the sums over m the m = phi terms, and in the contractions every
+  // term with one phi index, are products with an exact zero: they are left
+  // out (IEEE arithmetic keeps the compiler from dropping them itself), which
+  // changes no value, and only the symbols the contractions use are formed:
+  // Gam^k_{ij} for (i, j) in the rho-z block and (phi, phi).
+  constexpr int kIJ[5][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}, {2, 2}};
+  double Gam[2][3][3];
 #pragma unroll
   for (int k = 0; k < 2; ++k)
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int q = 0; q < 5; ++q) {
+      const int i = kIJ[q][0], j = kIJ[q][1];
+      double gm = zero;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double gm = zero;
-#pragma unroll
-        for (int m = 0; m < 3; ++m) gm = gm + half * gu[k][m] * (Dg[i][j][m] + Dg[j][i][m] - Dg[m][i][j]);
-        const double dki = (k == i) ? one : zero, dkj = (k == j) ? one : zero;
-        Gam[k][i][j] = gm + two * (dki * Dphi[j] + dkj * Dphi[i] - gd[i][j] * (gu[k][0] * Dphi[0] + gu[k][1] * Dphi[1]));
-      }
+      for (int m = 0; m < 2; ++m) gm = gm + half * gu[k][m] * (Dg[i][j][m] + Dg[j][i][m] - Dg[m][i][j]);
+      const double dki = (k == i) ? one : zero, dkj = (k == j) ? one : zero;
+      Gam[k][i][j] = gm + two * (dki * Dphi[j] + dkj * Dphi[i] - gd[i][j] * (gu[k][0] * Dphi[0] + gu[k][1] * Dphi[1]));
+    }
 
-  double s = zero;                                                               // :2615-2650
+  double s = zero;
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int q = 0; q < 5; ++q) {
+    const int i = kIJ[q][0], j = kIJ[q][1];
+    s = s + u2 * gu[i][j] * (Gam[0][i][j] - Gam[1][i][j] * dh) / interp_psi4;
+  }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) s = s + u2 * gu[i][j] * (Gam[0][i][j] - Gam[1][i][j] * dh) / interp_psi4;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) s = s - DF[i] * DF[j] * (Gam[0][i][j] - Gam[1][i][j] * dh);
+  for (int q = 0; q < 4; ++q) {
+    const int i = kIJ[q][0], j = kIJ[q][1];
+    s = s - DF[i] * DF[j] * (Gam[0][i][j] - Gam[1][i][j] * dh);
+  }
   s = s + u * u * u * v[AF_trK];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) s = s - u * DF[i] * DF[j] * Kd[i][j];
+  for (int q = 0; q < 4; ++q) {
+    const int i = kIJ[q][0], j = kIJ[q][1];
+    s = s - u * DF[i] * DF[j] * Kd[i][j];
+  }
   s = -s / denom;
   if (std::fabs(th) < 1.0e-5 || std::fabs(th - smallpi) < 1.0e-5) s = zero;    // :2663-2667
   if (s != s || std::fabs(s) > 1.0e50) {                                         // :2671-2676
