@@ -12,8 +12,9 @@ run to ahepsilon_spectral = 1e-10 so that one iteration more or less moves
 the surface by far less than that.  States: Schwarzschild (and a second
 search starting from the horizon found), equatorial symmetry, a horizon off
 the origin started from an ellipse, Schwarzschild through a coordinate
-change with ahmethod = both, ahshootfirst, flat space, an evolved state on
-two levels, a batch of searches and a begun search.
+change with ahmethod = both, ahshootfirst, 25 modes (the tables read from
+device memory), flat space, an evolved state on two levels, a batch of
+searches and a begun search.
 """
 import time
 
@@ -105,6 +106,17 @@ def test_shoot_first(oaxis_mod, oracle_mod):
     first, _ = both_find(st, orc, 0, "shoot first")
     second, _ = both_find(st, orc, 0, "after the first call")
     assert first["shots"] > 10 and second["shots"] == 0 and second["found"] == 1
+    st.close()
+    orc.close()
+
+
+def test_many_modes(oaxis_mod, oracle_mod):
+    """ahNmodes = 25 (Ntheta = 250): the cos / sin tables (104 KB) no longer
+    fit the kernel's LDS and are read from device memory, and the surface
+    takes two chunks of the source evaluation."""
+    st, orc = pair(oaxis_mod, oracle_mod, 0.1, schwarzschild_fields(2.0), extra=TIGHT + "ahNmodes = 25\n")
+    dev, _ = both_find(st, orc, 0, "25 modes")
+    assert dev["found"] == 1 and dev["ntheta"] == 250
     st.close()
     orc.close()
 
