@@ -152,9 +152,9 @@ __device__ __forceinline__ const Item& find_item(const Item* __restrict__ items,
 // update.  Kept apart, the phases need at most the registers of the larger
 // one.
 // Occupancy of the RHS kernel.  With the Gamma-driver or the scalar field the
-// staged tile (16 or 20 fields, 62-77 KB) allows two blocks per CU, and the
-// two phases fit 256 VGPRs: two waves per SIMD.  In vacuum the tile (12
-// fields, 46.5 KB) allows three blocks, and phase 1 split in two halves fits
+// staged tile (17 or 21 fields, 66-81 KB) allows two blocks per CU, and the
+// two phases fit 256 VGPRs: two waves per SIMD.  In vacuum the tile (13
+// fields, 50 KB) allows three blocks, and phase 1 split in two halves fits
 // 168 VGPRs: three waves per SIMD.
 template <bool SHIFT, bool SCALAR>
 constexpr bool kSplitPhase1 = !SHIFT && !SCALAR;
