@@ -428,35 +428,41 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   const double Dra = da.r, Dza = da.z, Drp = dp.r, Dzp = dp.z;
 
   // ---- geometric auxiliaries: D_i D_j alpha, the Ricci tensor, div KT.
-  // STAGE 0 computes them in place, STAGE 1 computes and stores them (one
-  // plane each in gaux, AUX_* order) and returns, STAGE 2 reads them back.
+  // STAGE 0 computes them in place, STAGE 1 computes the Ricci tensor and
+  // div KT and stores them (one plane each in gaux, AUX_* order) and returns,
+  // STAGE 2 reads them back.  D_i D_j alpha and its trace are formed where
+  // they are used (STAGE 0 and 2) from the lapse and metric derivatives that
+  // phase holds anyway, rather than passed through gaux.
   double d2A, d2B, d2C, Lapla, RA, RB, RC, RSCAL, DIV_r, DIV_z, DIVA_r, DIVA_z;
+  auto lapse_hessian = [&]() {
+    // D_i D_j alpha and its trace (:1268-1324)
+    d2A = da.rr - Dra * gC * r2 * dC.r
+                 - (Dza * gB * dC.r + 0.5 * Dza * gC * dA.r + Dra * gC * C - 0.5 * Dra * gC * dA.z) * r
+                 - 0.5 * Dra * gA * dA.r - Dza * gB * C + 0.5 * Dza * gB * dA.z;
+    d2B = da.zz - Dza * gC * r2 * dC.z
+                 - (Dra * gA * dC.z + 0.5 * Dra * gC * dB.z - 0.5 * Dza * gC * dB.r) * r
+                 + 0.5 * Dra * gA * dB.r - 0.5 * Dza * gB * dB.z;
+    double d2H = (0.5 * Dra * gC * dH.z + 0.5 * Dza * gC * dH.r) * r
+                 + 0.5 * Dra * gA * dH.r + Dza * gC * H + 0.5 * Dza * gB * dH.z + Dra * gA * H * ir;
+    d2C = da.rz * ir - 0.5 * Dra * gC * dB.r - 0.5 * Dza * gC * dA.z
+                 - (0.5 * Dra * gA * dA.z + 0.5 * Dza * gB * dB.r) * ir;
+    {
+      const double aux = gA * Dra * Drp + gB * Dza * Dzp + r * gC * (Dra * Dzp + Dza * Drp);
+      d2A -= 2.0 * (2.0 * Dra * Drp - A * aux);
+      d2B -= 2.0 * (2.0 * Dza * Dzp - B * aux);
+      d2H += 2.0 * H * aux;
+      d2C -= 2.0 * (Dra * ir * Dzp + Dza * Drp * ir - C * aux);
+    }
+    Lapla = (gA * d2A + gB * d2B + gH * d2H + 2.0 * r2 * gC * d2C) * ipsi4;
+  };
   if constexpr (STAGE == 2) {
     const double* const ax = gaux + k;
-    d2A = ax[AUX_d2A * pl]; d2B = ax[AUX_d2B * pl]; d2C = ax[AUX_d2C * pl]; Lapla = ax[AUX_Lapla * pl];
     RA = ax[AUX_RA * pl]; RB = ax[AUX_RB * pl]; RC = ax[AUX_RC * pl]; RSCAL = ax[AUX_RSCAL * pl];
     DIV_r = ax[AUX_DIV_r * pl]; DIV_z = ax[AUX_DIV_z * pl]; DIVA_r = ax[AUX_DIVA_r * pl]; DIVA_z = ax[AUX_DIVA_z * pl];
+    lapse_hessian();
   } else {
     if constexpr (STAGE != 12) {
-      // ---- D_i D_j alpha and its trace (:1268-1324)
-      d2A = da.rr - Dra * gC * r2 * dC.r
-                   - (Dza * gB * dC.r + 0.5 * Dza * gC * dA.r + Dra * gC * C - 0.5 * Dra * gC * dA.z) * r
-                   - 0.5 * Dra * gA * dA.r - Dza * gB * C + 0.5 * Dza * gB * dA.z;
-      d2B = da.zz - Dza * gC * r2 * dC.z
-                   - (Dra * gA * dC.z + 0.5 * Dra * gC * dB.z - 0.5 * Dza * gC * dB.r) * r
-                   + 0.5 * Dra * gA * dB.r - 0.5 * Dza * gB * dB.z;
-      double d2H = (0.5 * Dra * gC * dH.z + 0.5 * Dza * gC * dH.r) * r
-                   + 0.5 * Dra * gA * dH.r + Dza * gC * H + 0.5 * Dza * gB * dH.z + Dra * gA * H * ir;
-      d2C = da.rz * ir - 0.5 * Dra * gC * dB.r - 0.5 * Dza * gC * dA.z
-                   - (0.5 * Dra * gA * dA.z + 0.5 * Dza * gB * dB.r) * ir;
-      {
-        const double aux = gA * Dra * Drp + gB * Dza * Dzp + r * gC * (Dra * Dzp + Dza * Drp);
-        d2A -= 2.0 * (2.0 * Dra * Drp - A * aux);
-        d2B -= 2.0 * (2.0 * Dza * Dzp - B * aux);
-        d2H += 2.0 * H * aux;
-        d2C -= 2.0 * (Dra * ir * Dzp + Dza * Drp * ir - C * aux);
-      }
-      Lapla = (gA * d2A + gB * d2B + gH * d2H + 2.0 * r2 * gC * d2C) * ipsi4;
+      if constexpr (STAGE == 0) lapse_hessian();
 
       // ---- Ricci tensor (:1397-1493 and calc_conformalRicci1..4, non-angmom)
       double RH;
@@ -598,7 +604,6 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
   if constexpr (STAGE == 1 || STAGE == 11 || STAGE == 12) {
     double* const ax = gaux + k;
     if constexpr (STAGE != 12) {
-      ax[AUX_d2A * pl] = d2A; ax[AUX_d2B * pl] = d2B; ax[AUX_d2C * pl] = d2C; ax[AUX_Lapla * pl] = Lapla;
       ax[AUX_RA * pl] = RA; ax[AUX_RB * pl] = RB; ax[AUX_RC * pl] = RC; ax[AUX_RSCAL * pl] = RSCAL;
     }
     if constexpr (STAGE != 11) {
