@@ -23,13 +23,11 @@ enum ShiftSlot { SH_beta_r = 0, SH_beta_z, SH_dtbeta_r, SH_dtbeta_z, NSHIFT };
 static_assert(SH_dtbeta_z == SH_dtbeta_r + 1, "dtbeta_r and dtbeta_z planes are adjacent");
 enum ScalarSlot { SC_phi = 0, SC_xi_r, SC_xi_z, SC_pi, NSCALAR };
 
-// Planes of the per-level buffer of geometric auxiliaries that the first RHS
-// kernel of a substep writes and the second reads: D_i D_j alpha (A, B, C
-// components and the Laplacian), the Ricci tensor (A, B, C components and the
-// scalar), and the divergences DIV_KT, DIVA_KT.
+// Planes of the per-level buffer of geometric auxiliaries that the first
+// phase of the RHS kernel writes and the second reads: the Ricci tensor (A,
+// B, C components and the scalar) and the divergences DIV_KT, DIVA_KT.
 enum AuxSlot {
-  AUX_d2A = 0, AUX_d2B, AUX_d2C, AUX_Lapla, AUX_RA, AUX_RB, AUX_RC, AUX_RSCAL,
-  AUX_DIV_r, AUX_DIV_z, AUX_DIVA_r, AUX_DIVA_z, NAUX
+  AUX_RA = 0, AUX_RB, AUX_RC, AUX_RSCAL, AUX_DIV_r, AUX_DIV_z, AUX_DIVA_r, AUX_DIVA_z, NAUX
 };
 
 constexpr int nslots(bool shift, bool scalar) {

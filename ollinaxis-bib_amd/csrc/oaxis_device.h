@@ -599,8 +599,8 @@ OA_HD void rhs_point(const KGrid& G, const KPhys& P, const KStep& S, const View&
       }
     }
   }
-  // STAGE 11 / 12: the two halves of STAGE 1 (D_i D_j alpha and the Ricci
-  // tensor; div KT), for kernels that run them one after the other
+  // STAGE 11 / 12: the two halves of STAGE 1 (the Ricci tensor; div KT),
+  // for kernels that run them one after the other
   if constexpr (STAGE == 1 || STAGE == 11 || STAGE == 12) {
     double* const ax = gaux + k;
     if constexpr (STAGE != 12) {

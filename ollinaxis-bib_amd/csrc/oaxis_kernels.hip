@@ -145,12 +145,12 @@ __device__ __forceinline__ const Item& find_item(const Item* __restrict__ items,
 }
 
 // One RK substep of a level: each 16 x 16 block stages its tile and runs two
-// phases per point.  Phase 1 forms the geometric auxiliaries (D_i D_j alpha,
-// the Ricci tensor, div KT: the longest expressions, which need only the
-// vacuum fields) and passes them through the per-level buffer gaux; phase 2
-// forms every source, the dissipation, the radiative boundaries and the RK
-// update.  Kept apart, the phases need at most the registers of the larger
-// one.
+// phases per point.  Phase 1 forms the geometric auxiliaries (the Ricci
+// tensor and div KT: the longest expressions, which need only the vacuum
+// fields) and passes them through the per-level buffer gaux; phase 2 forms
+// D_i D_j alpha, every source, the dissipation, the radiative boundaries and
+// the RK update.  Kept apart, the phases need at most the registers of the
+// larger one.
 // Occupancy of the RHS kernel.  With the Gamma-driver or the scalar field the
 // staged tile (17 or 21 fields, 66-81 KB) allows two blocks per CU, and the
 // two phases fit 256 VGPRs: two waves per SIMD.  In vacuum the tile (13
@@ -189,7 +189,7 @@ __device__ __forceinline__ void rhs_tile(const RhsItem& it, int bi, int bj, doub
   // carrying phase-1 values or loads into phase 2, so the two phases do not
   // add up in registers
   if constexpr (kSplitPhase1<SHIFT, SCALAR>) {
-    // vacuum: phase 1 in its two halves (D_i D_j alpha + Ricci, then div KT),
+    // vacuum: phase 1 in its two halves (the Ricci tensor, then div KT),
     // so that no part needs more than 168 VGPRs: three waves per SIMD
     if (interior_tile(G, i0, j0)) {
       rhs_point<false, false, PHIMODE, true, 11>(G, it.P, S, V, nullptr, nullptr, nullptr, gaux, i, j);
