@@ -187,11 +187,14 @@ __global__ void __launch_bounds__(256) k_ah_fields(KGrid G, const double* __rest
 // registers (AhCell): the adaptive RK4 of a shot takes many small steps, so
 // consecutive evaluations mostly fall in the same cell of the same level and
 // reuse them instead of waiting for another round of global loads.  The
-// values, and so the results, are the same either way.
+// values, and so the results, are the same either way.  So do the sine and
+// cosine of the last angle (stages 2 and 3 of an RK4 step, and the first
+// evaluation after an accepted step, repeat the angle before them).
 struct AhCell {
   const double* q;       // level records the cell belongs to (null: empty)
   int i0, j0;            // its lower-left stencil point + (1, 1)
   double fa[4][4];       // quantity f of the records (i0-1+a, j0-1+b), indices clamped
+  double th, sth, cth;   // the last angle (NaN: none) and its sine and cosine
 };
 
 // How interp.f90 combines the 4 x 4 stencil records around (r0, z0) on one
@@ -313,13 +316,14 @@ struct AhPos {
   int lev, err;
 };
 
-__device__ AhPos ah_locate(const AhCtx& c, double rad, double th, double zcenter) {
+// sth, cth: the sine and cosine of the angle
+__device__ AhPos ah_locate_sc(const AhCtx& c, double rad, double sth, double cth, double zcenter) {
   const AhLevel& L0 = c.L[0];
   // the checks that end the reference's ah_source early set err; the
   // interpolation point then moves to the middle of level 0
   int err = (rad < 0.0) ? 1 : 0;                                                 // :2140-2145
-  const double rc = rad * std::sin(th);                                          // :2151-2159
-  const double zc = rad * std::cos(th);
+  const double rc = rad * sth;                                                   // :2151-2159
+  const double zc = rad * cth;
   double r0 = rc;
   double z0 = zc + zcenter;
   if (c.eqsym && z0 < 0.0) z0 = -z0;
@@ -348,6 +352,10 @@ __device__ AhPos ah_locate(const AhCtx& c, double rad, double th, double zcenter
   P.lev = lev;
   P.err = err;
   return P;
+}
+
+__device__ AhPos ah_locate(const AhCtx& c, double rad, double th, double zcenter) {
+  return ah_locate_sc(c, rad, std::sin(th), std::cos(th), zcenter);
 }
 
 __device__ int ah_source_math(double rad, double dh, double th, double rc, double zc, const double (&v)[AF_N],
@@ -504,7 +512,12 @@ __device__ int ah_source_math(double rad, double dh, double th, double rc, doubl
 __device__ int ah_source(const AhCtx& c, double rad, double dh, double th, double zcenter, double& sdh,
                          double& sarea, double* sv, AhCell& cell) {
   sdh = 0.0;
-  const AhPos P = ah_locate(c, rad, th, zcenter);
+  if (!(th == cell.th)) {
+    cell.th = th;
+    cell.sth = std::sin(th);
+    cell.cth = std::cos(th);
+  }
+  const AhPos P = ah_locate_sc(c, rad, cell.sth, cell.cth, zcenter);
   double v[AF_N];
   ah_interp(c.L[P.lev], c.interporder, P.r0, P.z0, v, sv, cell);
   if (P.err) return 1;
@@ -545,6 +558,7 @@ __device__ ShotOut ah_integrate(const AhCtx& c, double zinit, double zcenter, do
   AhCell cell;
   cell.q = nullptr;
   cell.i0 = cell.j0 = 0;
+  cell.th = __builtin_nan("");
   for (int steps = 0; std::fabs(theta - thetafinal) > 1.0e-3; ++steps) {
     if (steps >= kAhMaxSteps) {
       err = 1;
@@ -682,6 +696,7 @@ __global__ void __launch_bounds__(kShotBlock) k_ah_dzphi(const AhCtx* __restrict
   AhCell cell;
   cell.q = nullptr;
   cell.i0 = cell.j0 = 0;
+  cell.th = __builtin_nan("");
   ah_interp(c->L[level], c->interporder, r0, z0, v, sv, cell);
   if (threadIdx.x == 0) out[0] = v[AF_Dzphi];
 }
